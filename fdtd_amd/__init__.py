@@ -1,0 +1,271 @@
+"""fdtd_amd -- MI355X-native FDTD time-stepping path (Yee H, D from curl H,
+Debye E, first-order Mur, soft Jz sources, probes and z-slices).
+
+Python binding of libfdtd_amd.so, whose C ABI is declared in include/:
+  * fdtd_calculations.h -- the reference's device entry points
+    (copySymbolsToDevice, updateHField, updateDField, updateEField,
+    updateSources, updateMurBoundary), fdtd_calculations.h:18-42;
+  * fdtd.h -- FdtdParams / FdtdField and the host services
+    (initParamsWithPath, setupSources, setupMurBoundary, loadMaterials, ...);
+  * fdtd_amd.h -- the context API with the fused gfx950 step.
+
+The library must be built (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C fdtd_amd/csrc``); there is no CPU fallback: every compute call
+goes to the HIP kernels and fails loudly when the library or a GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfdtd_amd.so")
+CLI_PATH = os.path.join(_HERE, "fdtd")
+
+DEBYE_REF = 0
+DEBYE_FIXED = 1
+JZ_HOLD3 = 0
+JZ_STEP = 1
+FIELDS = ["Dx", "Dy", "Dz", "Hx", "Hy", "Hz", "Ex", "Ey", "Ez"]
+
+# Every symbol the headers in include/ declare.
+EXPORTS = {
+    "fdtd_calculations.h": ["copySymbolsToDevice", "updateHField", "updateDField", "updateEField",
+                            "updateSources", "updateMurBoundary", "fdtdSetDebyeMode"],
+    "fdtd.h": ["initParamsWithPath", "deallocParams", "printParams", "initFieldWithParams", "deallocField",
+               "initDeviceFieldWithParams", "deallocDeviceField", "setupMurBoundary", "loadMaterials",
+               "setupSources", "copyDataToDevice", "copyResultsWithParams", "writeResultsWithParams",
+               "writeResults", "writeResults3d"],
+    "fdtd_amd.h": ["fdtd_last_error", "fdtd_ext_defaults", "fdtd_ctx_create", "fdtd_ctx_destroy",
+                   "fdtd_ctx_set_materials", "fdtd_ctx_set_material_arrays", "fdtd_ctx_set_rp",
+                   "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
+                   "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
+                   "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
+                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time"],
+}
+
+
+class FdtdParams(C.Structure):
+    """fdtd.h:7-34, field for field."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("iterationsCount", C.c_int),
+        ("inputPath", C.c_char_p), ("outputPath", C.c_char_p),
+        ("elementsPerWave", C.c_int),
+        ("waveFrequency", C.c_float), ("pulseWidth", C.c_float), ("pulseModulationFrequency", C.c_float),
+        ("sourcesCount", C.c_int), ("sources", C.POINTER(C.c_int)),
+        ("defaultSigma", C.c_float), ("defaultEpsS", C.c_float), ("defaultEpsI", C.c_float),
+        ("defaultTauD", C.c_float),
+        ("jzCount", C.c_int), ("jz", C.POINTER(C.c_float)),
+        ("pi", C.c_float), ("c", C.c_float), ("timeskip", C.c_float), ("lambda_", C.c_float),
+        ("dt", C.c_float), ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+        ("mu0", C.c_float), ("eps0", C.c_float),
+    ]
+
+
+_FP = C.POINTER(C.c_float)
+
+
+class FdtdField(C.Structure):
+    """fdtd.h:37-54."""
+    _fields_ = [(n, _FP) for n in (
+        "ex0", "ey0", "ez0", "ex1", "ey1", "ez1", "ex2", "ey2", "ez2", "hx", "hy", "hz",
+        "dx0", "dy0", "dz0", "dx1", "dy1", "dz1", "dx2", "dy2", "dz2",
+        "epsI", "epsS", "tauD", "sigma", "rpx0", "rpy0", "rpz0", "rpxEnd", "rpyEnd", "rpzEnd")]
+
+
+class FdtdExt(C.Structure):
+    _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
+                ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int)]
+
+
+class FdtdError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libfdtd_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FdtdError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                            "or `make -C fdtd_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, i = C.c_void_p, C.c_int, C.c_int
+        L.fdtd_last_error.restype = C.c_char_p
+        L.fdtd_ext_defaults.argtypes = [C.POINTER(FdtdExt)]
+        L.fdtd_ctx_create.argtypes = [C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(vp)]
+        L.fdtd_ctx_destroy.argtypes = [vp]
+        L.fdtd_ctx_set_materials.argtypes = [vp, vp, i, vp]
+        L.fdtd_ctx_set_material_arrays.argtypes = [vp, vp, vp, vp, vp]
+        L.fdtd_ctx_set_rp.argtypes = [vp] * 7
+        L.fdtd_ctx_set_sources.argtypes = [vp, i, vp, i, vp]
+        L.fdtd_ctx_set_probes.argtypes = [vp, i, vp, i]
+        L.fdtd_ctx_set_slices.argtypes = [vp, i, i, i]
+        L.fdtd_ctx_set_field.argtypes = [vp, i, i, vp]
+        L.fdtd_ctx_get_field.argtypes = [vp, i, i, vp]
+        L.fdtd_ctx_field_ptr.argtypes = [vp, i, i, C.POINTER(vp)]
+        L.fdtd_step_fused.argtypes = [vp, vp]
+        L.fdtd_run.argtypes = [vp, i, vp]
+        L.fdtd_ctx_steps_done.argtypes = [vp]
+        L.fdtd_probe_read.argtypes = [vp, vp, i]
+        L.fdtd_slices_read.argtypes = [vp, vp, vp, i]
+        L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
+        L.fdtd_ctx_set_timing.argtypes = [vp, i]
+        L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.copySymbolsToDevice.argtypes = [C.POINTER(FdtdParams)]
+        L.updateHField.argtypes = [vp] * 7
+        L.updateDField.argtypes = [vp] * 10
+        L.updateEField.argtypes = [vp] * 23
+        L.updateSources.argtypes = [vp, vp, vp, vp, i, vp]
+        L.updateMurBoundary.argtypes = [vp] * 13
+        L.fdtdSetDebyeMode.argtypes = [i]
+        L.initParamsWithPath.argtypes = [C.c_char_p]
+        L.initParamsWithPath.restype = C.POINTER(FdtdParams)
+        L.deallocParams.argtypes = [C.POINTER(FdtdParams)]
+        L.setupSources.argtypes = [C.POINTER(FdtdParams)]
+        L.initFieldWithParams.argtypes = [C.POINTER(FdtdParams)]
+        L.initFieldWithParams.restype = C.POINTER(FdtdField)
+        L.deallocField.argtypes = [C.POINTER(FdtdField)]
+        L.setupMurBoundary.argtypes = [C.POINTER(FdtdParams), C.POINTER(FdtdField)]
+        L.loadMaterials.argtypes = [C.POINTER(FdtdParams), C.POINTER(FdtdField), C.c_char_p, C.c_char_p]
+        L.writeResults.argtypes = [C.POINTER(FdtdParams)] + [vp] * 9 + [i]
+        L.writeResults3d.argtypes = [C.POINTER(FdtdParams)] + [vp] * 9 + [i]
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().fdtd_last_error().decode(errors="replace")
+        raise FdtdError(f"{what} failed with code {rc}: {msg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_params(shape, dt, dx, dy=None, dz=None, mu0=None, eps0=None):
+    """An FdtdParams with the scalars the device path uses (others zero)."""
+    nx, ny, nz = shape
+    p = FdtdParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.dt = dt
+    p.dx = dx
+    p.dy = dx if dy is None else dy
+    p.dz = dx if dz is None else dz
+    p.mu0 = mu0
+    p.eps0 = eps0
+    return p
+
+
+class Context:
+    """One grid's device state and the fused step (fdtd_amd.h)."""
+
+    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=1):
+        L = lib()
+        ext = FdtdExt()
+        L.fdtd_ext_defaults(C.byref(ext))
+        ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
+        ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
+        h = C.c_void_p()
+        self._params = params
+        check(L.fdtd_ctx_create(C.byref(params), C.byref(ext), C.byref(h)), "fdtd_ctx_create")
+        self._h = h
+        self.shape = (params.nx, params.ny, params.nz)
+        self.nprobe = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_materials(self, ids, table):
+        ids = np.ascontiguousarray(ids, np.uint8).ravel()
+        table = np.ascontiguousarray(table, np.float32).reshape(-1, 4)
+        check(lib().fdtd_ctx_set_materials(self._h, _ptr(ids), table.shape[0], _ptr(table)), "set_materials")
+
+    def set_material_arrays(self, sigma, epsS, epsI, tau):
+        a = [np.ascontiguousarray(x, np.float32).ravel() for x in (sigma, epsS, epsI, tau)]
+        check(lib().fdtd_ctx_set_material_arrays(self._h, *[_ptr(x) for x in a]), "set_material_arrays")
+
+    def set_rp(self, rps):
+        a = [np.ascontiguousarray(x, np.float32).ravel() for x in rps]
+        check(lib().fdtd_ctx_set_rp(self._h, *[_ptr(x) for x in a]), "set_rp")
+
+    def set_sources(self, xyz, jz):
+        xyz = np.ascontiguousarray(np.asarray(xyz, np.int32).reshape(-1, 3))
+        jz = np.ascontiguousarray(jz, np.float32)
+        check(lib().fdtd_ctx_set_sources(self._h, xyz.shape[0], _ptr(xyz), jz.size, _ptr(jz)), "set_sources")
+
+    def set_probes(self, probes, capacity):
+        pr = np.ascontiguousarray(np.asarray(probes, np.int32).reshape(-1, 4))
+        self.nprobe = pr.shape[0]
+        check(lib().fdtd_ctx_set_probes(self._h, pr.shape[0], _ptr(pr), capacity), "set_probes")
+
+    def set_slices(self, k, interval, capacity):
+        check(lib().fdtd_ctx_set_slices(self._h, k, interval, capacity), "set_slices")
+
+    def set(self, field, level, arr):
+        fid = FIELDS.index(field) if isinstance(field, str) else field
+        a = np.ascontiguousarray(arr, np.float32).ravel()
+        check(lib().fdtd_ctx_set_field(self._h, fid, level, _ptr(a)), "set_field")
+
+    def get(self, field, level=0):
+        fid = FIELDS.index(field) if isinstance(field, str) else field
+        nx, ny, nz = self.shape
+        out = np.empty(nx * ny * nz, np.float32)
+        check(lib().fdtd_ctx_get_field(self._h, fid, level, _ptr(out)), "get_field")
+        return out.reshape(nz, ny, nx)
+
+    def field_ptr(self, field, level=0):
+        fid = FIELDS.index(field) if isinstance(field, str) else field
+        p = C.c_void_p()
+        check(lib().fdtd_ctx_field_ptr(self._h, fid, level, C.byref(p)), "field_ptr")
+        return p.value
+
+    def step(self, stream=None):
+        check(lib().fdtd_step_fused(self._h, stream), "fdtd_step_fused")
+
+    def run(self, steps, stream=None):
+        check(lib().fdtd_run(self._h, steps, stream), "fdtd_run")
+
+    @property
+    def n(self):
+        return lib().fdtd_ctx_steps_done(self._h)
+
+    def probes(self, max_rows=None):
+        rows = self.n if max_rows is None else max_rows
+        out = np.zeros((max(rows, 1), max(self.nprobe, 1)), np.float64)
+        got = lib().fdtd_probe_read(self._h, _ptr(out), rows)
+        if got < 0:
+            check(got, "probe_read")
+        return out[:got, :self.nprobe]
+
+    def slices(self, max_slices=1024):
+        nx, ny, _ = self.shape
+        steps = np.zeros(max_slices, np.int32)
+        data = np.zeros((max_slices, 9, ny, nx), np.float32)
+        got = lib().fdtd_slices_read(self._h, _ptr(steps), _ptr(data), max_slices)
+        if got < 0:
+            check(got, "slices_read")
+        return steps[:got], data[:got]
+
+    def set_timing(self, capacity):
+        check(lib().fdtd_ctx_set_timing(self._h, capacity), "set_timing")
+
+    def kernel_time(self):
+        """(summed fused-kernel milliseconds, launches timed)."""
+        ms, n = C.c_double(), C.c_int()
+        check(lib().fdtd_ctx_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def launch_shape(self):
+        v = [C.c_int() for _ in range(4)]
+        check(lib().fdtd_ctx_launch_shape(self._h, *[C.byref(x) for x in v]), "launch_shape")
+        return dict(tile_x=v[0].value, tile_y=v[1].value, kchunk=v[2].value, blocks=v[3].value)

@@ -1,0 +1,530 @@
+// Context-based API of libfdtd_amd (fdtd_amd.h): device state of one grid,
+// the fused step sequence (fused H/D/E kernel -> Mur faces -> probes) and
+// the overlapped z-slice capture on a side stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fdtd_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail((int)e, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+#define HIP_TRY(call)                                    \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+}  // namespace
+
+MatCoef fdtd_mat_coef(float dDt, float dEps0, float sg, float es, float ei, float tau, int debye_mode)
+{
+    // Same expressions and precisions as fdtd_calculations.cu:141-155.
+    MatCoef m;
+    m.a = 2.0 * dEps0 * ei * tau;
+    const float b = m.a + 2.0 * dDt * (dEps0 * es + sg * tau);
+    m.c = 1.0 / (b + sg * dDt * dDt);
+    m.d = (debye_mode == FDTD_DEBYE_REF) ? (float)(2.0 * b - sg * dDt * dDt)
+                                         : (float)((double)m.a + (double)b - (double)(sg * dDt * dDt));
+    m.pad = 0.0f;
+    m.g1 = 2.0 * (dDt + tau);
+    m.g2 = 2.0 * dDt + 4.0 * tau;
+    m.g3 = 2.0 * tau;
+    return m;
+}
+
+struct fdtd_ctx {
+    int nx, ny, nz;
+    size_t N;
+    int pitch;
+    long long plane;
+    float dt, dx, dy, dz, mu0, eps0;
+    FdtdExt ext;
+    int ty, kchunk, tiles_x, tiles_y, tiles_z;
+
+    float *blob = nullptr;               // all field arrays
+    float *E[3][3], *D[3][3], *H[2][3];  // [slot][component], H[buffer][component]
+    int hcur = 0;                        // H buffer holding the newest H
+    unsigned char *ids = nullptr;
+    MatCoef *lut = nullptr;
+    int nmat = 0;
+    float *rp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<unsigned char> ids_host;  // material ids without source bits
+    std::vector<int> src;                 // 3 per source
+    std::vector<float> jz;
+
+    int *probes = nullptr;
+    double *probe_out = nullptr;
+    int nprobe = 0, probe_cap = 0, probe_rows = 0;
+
+    // slices
+    int slice_k = -1, slice_interval = 0, slice_cap = 0, slice_count = 0;
+    float *slice_stage = nullptr;   // device: cap x 9 x nx*ny
+    float *slice_host = nullptr;    // pinned host: cap x 9 x nx*ny
+    std::vector<int> slice_steps;
+    hipStream_t side = nullptr;
+    hipEvent_t slice_ev = nullptr;
+
+    std::vector<hipEvent_t> tev;  // start/end pairs around fused launches
+    int tcap = 0, tused = 0;
+
+    int n = 0;  // steps done
+};
+
+namespace {
+
+int slot_of(const fdtd_ctx *c, int level) { return ((c->n + 2 - level) % 3 + 3) % 3; }
+
+float *field_dev(fdtd_ctx *c, int field, int level)
+{
+    const int q = field % 3;
+    if (field >= FDTD_HX && field <= FDTD_HZ) return c->H[c->hcur][q];
+    const int s = slot_of(c, level);
+    return field < 3 ? c->D[s][q] : c->E[s][q];
+}
+
+int upload_ids(fdtd_ctx *c)
+{
+    if (c->ids_host.empty()) return 0;
+    std::vector<unsigned char> merged(c->ids_host);
+    const int nx = c->nx, ny = c->ny;
+    for (size_t s = 0; s < c->src.size() / 3; s++) {
+        const size_t o = ((size_t)c->src[3 * s + 2] * ny + c->src[3 * s + 1]) * nx + c->src[3 * s];
+        merged[o] |= kSourceBit;
+    }
+    HIP_TRY(hipMemcpy(c->ids, merged.data(), c->N, hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" const char *fdtd_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fdtd_ext_defaults(FdtdExt *ext)
+{
+    if (!ext) return FDTD_EINVAL;
+    ext->debye_mode = FDTD_DEBYE_FIXED;
+    ext->jz_index_mode = FDTD_JZ_STEP;
+    ext->jz_limit = 0;
+    ext->tile_y = 0;
+    ext->kchunk = 0;
+    ext->xcd_remap = 1;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx **out)
+{
+    if (!p || !out) return fail(FDTD_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
+    const long long N = (long long)p->nx * p->ny * p->nz;
+    // The fused kernel addresses each field array with 32-bit byte offsets.
+    if (N >= (1LL << 30)) return fail(FDTD_ERANGE, "grid of %lld cells: at most 2^30 - 1 per context", N);
+    FdtdExt e;
+    fdtd_ext_defaults(&e);
+    if (ext) e = *ext;
+    if (e.debye_mode != FDTD_DEBYE_REF && e.debye_mode != FDTD_DEBYE_FIXED) return fail(FDTD_EINVAL, "debye_mode %d", e.debye_mode);
+    if (e.jz_index_mode != FDTD_JZ_STEP && e.jz_index_mode != FDTD_JZ_HOLD3) return fail(FDTD_EINVAL, "jz_index_mode %d", e.jz_index_mode);
+    if (e.tile_y != 0 && e.tile_y != 1 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
+    if (e.kchunk < 0 || e.jz_limit < 0) return fail(FDTD_EINVAL, "negative kchunk/jz_limit");
+
+    fdtd_ctx *c = new fdtd_ctx();
+    c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
+    c->N = (size_t)N;
+    c->pitch = p->nx;
+    c->plane = (long long)p->nx * p->ny;
+    c->dt = p->dt; c->dx = p->dx; c->dy = p->dy; c->dz = p->dz; c->mu0 = p->mu0; c->eps0 = p->eps0;
+    c->ext = e;
+    c->ty = e.tile_y ? e.tile_y : 4;
+    c->tiles_x = (c->nx + 63) / 64;
+    c->tiles_y = (c->ny + c->ty - 1) / c->ty;
+    if (e.kchunk) {
+        c->kchunk = std::min(e.kchunk, c->nz);
+    } else {
+        const long long target = 4096;
+        const long long txy = (long long)c->tiles_x * c->tiles_y;
+        long long kc = (c->nz * txy + target - 1) / target;
+        c->kchunk = (int)std::max(4LL, std::min<long long>(kc, c->nz));
+    }
+    c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
+
+    const size_t nf = 24;  // 9 E + 9 D + 6 H
+    hipError_t he = hipMalloc(&c->blob, nf * c->N * sizeof(float));
+    if (he != hipSuccess) { delete c; return hip_fail(he, "hipMalloc(fields)"); }
+    float *q = c->blob;
+    for (int s = 0; s < 3; s++)
+        for (int k = 0; k < 3; k++) { c->E[s][k] = q; q += c->N; }
+    for (int s = 0; s < 3; s++)
+        for (int k = 0; k < 3; k++) { c->D[s][k] = q; q += c->N; }
+    for (int s = 0; s < 2; s++)
+        for (int k = 0; k < 3; k++) { c->H[s][k] = q; q += c->N; }
+    he = hipMemset(c->blob, 0, nf * c->N * sizeof(float));
+    if (he == hipSuccess) he = hipMalloc(&c->ids, c->N);
+    if (he == hipSuccess) he = hipMemset(c->ids, 0, c->N);
+    if (he == hipSuccess) he = hipMalloc(&c->lut, sizeof(MatCoef) * kMaxMaterials);
+    const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
+                           (size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2};
+    for (int r = 0; r < 6 && he == hipSuccess; r++) {
+        he = hipMalloc(&c->rp[r], rps[r] * sizeof(float));
+        if (he == hipSuccess) {
+            std::vector<float> ones(rps[r], 1.0f);
+            he = hipMemcpy(c->rp[r], ones.data(), rps[r] * sizeof(float), hipMemcpyHostToDevice);
+        }
+    }
+    if (he != hipSuccess) {
+        int rc = hip_fail(he, "fdtd_ctx_create");
+        fdtd_ctx_destroy(c);
+        return rc;
+    }
+    // Vacuum default material so a context steps before materials are set.
+    MatCoef vac = fdtd_mat_coef(c->dt, c->eps0, 0.0f, 1.0f, 1.0f, 0.0f, e.debye_mode);
+    he = hipMemcpy(c->lut, &vac, sizeof vac, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        int rc = hip_fail(he, "fdtd_ctx_create");
+        fdtd_ctx_destroy(c);
+        return rc;
+    }
+    c->nmat = 1;
+    *out = c;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
+{
+    if (!c) return 0;
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    (void)hipFree(c->blob);
+    (void)hipFree(c->ids);
+    (void)hipFree(c->lut);
+    for (int r = 0; r < 6; r++) (void)hipFree(c->rp[r]);
+    (void)hipFree(c->probes);
+    (void)hipFree(c->probe_out);
+    (void)hipFree(c->slice_stage);
+    if (c->slice_host) (void)hipHostFree(c->slice_host);
+    if (c->slice_ev) (void)hipEventDestroy(c->slice_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    for (auto ev : c->tev) (void)hipEventDestroy(ev);
+    delete c;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_set_materials(fdtd_ctx *c, const unsigned char *ids, int nmat, const float *table)
+{
+    if (!c || !ids || !table) return fail(FDTD_EINVAL, "null argument");
+    if (nmat < 1 || nmat > kMaxMaterials) return fail(FDTD_ERANGE, "nmat %d not in [1,%d]", nmat, kMaxMaterials);
+    for (size_t t = 0; t < c->N; t++)
+        if (ids[t] >= nmat) return fail(FDTD_ERANGE, "material id %d at cell %zu >= nmat %d", ids[t], t, nmat);
+    std::vector<MatCoef> lut(nmat);
+    for (int m = 0; m < nmat; m++)
+        lut[m] = fdtd_mat_coef(c->dt, c->eps0, table[4 * m], table[4 * m + 1], table[4 * m + 2], table[4 * m + 3],
+                               c->ext.debye_mode);
+    HIP_TRY(hipMemcpy(c->lut, lut.data(), sizeof(MatCoef) * nmat, hipMemcpyHostToDevice));
+    c->nmat = nmat;
+    c->ids_host.assign(ids, ids + c->N);
+    return upload_ids(c);
+}
+
+extern "C" int fdtd_ctx_set_material_arrays(fdtd_ctx *c, const float *sigma, const float *epsS, const float *epsI,
+                                            const float *tauD)
+{
+    if (!c || !sigma || !epsS || !epsI || !tauD) return fail(FDTD_EINVAL, "null argument");
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, int> seen;
+    std::vector<float> table;
+    std::vector<unsigned char> ids(c->N);
+    for (size_t t = 0; t < c->N; t++) {
+        uint32_t b[4];
+        memcpy(&b[0], &sigma[t], 4); memcpy(&b[1], &epsS[t], 4); memcpy(&b[2], &epsI[t], 4); memcpy(&b[3], &tauD[t], 4);
+        auto key = std::make_tuple(b[0], b[1], b[2], b[3]);
+        auto it = seen.find(key);
+        int id;
+        if (it == seen.end()) {
+            id = (int)seen.size();
+            if (id >= kMaxMaterials) return fail(FDTD_ERANGE, "more than %d distinct materials", kMaxMaterials);
+            seen.emplace(key, id);
+            table.insert(table.end(), {sigma[t], epsS[t], epsI[t], tauD[t]});
+        } else {
+            id = it->second;
+        }
+        ids[t] = (unsigned char)id;
+    }
+    return fdtd_ctx_set_materials(c, ids.data(), (int)seen.size(), table.data());
+}
+
+extern "C" int fdtd_ctx_set_rp(fdtd_ctx *c, const float *rpx0, const float *rpy0, const float *rpz0,
+                               const float *rpxEnd, const float *rpyEnd, const float *rpzEnd)
+{
+    if (!c) return fail(FDTD_EINVAL, "null context");
+    const float *src[6] = {rpx0, rpy0, rpz0, rpxEnd, rpyEnd, rpzEnd};
+    const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
+                           (size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2};
+    for (int r = 0; r < 6; r++) {
+        if (!src[r]) return fail(FDTD_EINVAL, "null rp plane %d", r);
+        HIP_TRY(hipMemcpy(c->rp[r], src[r], rps[r] * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+extern "C" int fdtd_ctx_set_sources(fdtd_ctx *c, int nsrc, const int *xyz, int jz_len, const float *jz)
+{
+    if (!c || nsrc < 0 || jz_len < 0 || (nsrc > 0 && !xyz) || (jz_len > 0 && !jz)) return fail(FDTD_EINVAL, "bad sources");
+    for (int s = 0; s < nsrc; s++) {
+        const int x = xyz[3 * s], y = xyz[3 * s + 1], z = xyz[3 * s + 2];
+        if (x < 1 || x >= c->nx - 1 || y < 1 || y >= c->ny - 1 || z < 0 || z >= c->nz - 1)
+            return fail(FDTD_ERANGE, "source %d at (%d,%d,%d) outside the Dz update range", s, x, y, z);
+    }
+    if (c->ids_host.empty()) c->ids_host.assign(c->N, 0);
+    c->src.assign(xyz, xyz + 3 * nsrc);
+    c->jz.assign(jz, jz + jz_len);
+    return upload_ids(c);
+}
+
+extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, int capacity)
+{
+    if (!c || nprobe < 0 || capacity < 0 || (nprobe > 0 && !probes)) return fail(FDTD_EINVAL, "bad probes");
+    for (int p = 0; p < nprobe; p++) {
+        const int *q = probes + 4 * p;
+        if (q[0] < 0 || q[0] > 8 || q[1] < 0 || q[1] >= c->nx || q[2] < 0 || q[2] >= c->ny || q[3] < 0 || q[3] >= c->nz)
+            return fail(FDTD_ERANGE, "probe %d out of range", p);
+    }
+    (void)hipFree(c->probes);
+    (void)hipFree(c->probe_out);
+    c->probes = nullptr;
+    c->probe_out = nullptr;
+    c->nprobe = nprobe;
+    c->probe_cap = capacity;
+    c->probe_rows = 0;
+    if (nprobe == 0 || capacity == 0) { c->nprobe = 0; return 0; }
+    HIP_TRY(hipMalloc(&c->probes, sizeof(int) * 4 * nprobe));
+    HIP_TRY(hipMemcpy(c->probes, probes, sizeof(int) * 4 * nprobe, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->probe_out, sizeof(double) * (size_t)nprobe * capacity));
+    HIP_TRY(hipMemset(c->probe_out, 0, sizeof(double) * (size_t)nprobe * capacity));
+    return 0;
+}
+
+extern "C" int fdtd_ctx_set_slices(fdtd_ctx *c, int k, int interval, int capacity)
+{
+    if (!c || interval < 0 || capacity < 0) return fail(FDTD_EINVAL, "bad slices");
+    if (interval > 0 && (k < 0 || k >= c->nz)) return fail(FDTD_ERANGE, "slice plane %d out of range", k);
+    if (c->side) HIP_TRY(hipStreamSynchronize(c->side));
+    (void)hipFree(c->slice_stage);
+    if (c->slice_host) (void)hipHostFree(c->slice_host);
+    c->slice_stage = nullptr;
+    c->slice_host = nullptr;
+    c->slice_k = k;
+    c->slice_interval = interval;
+    c->slice_cap = capacity;
+    c->slice_count = 0;
+    c->slice_steps.clear();
+    if (interval == 0 || capacity == 0) { c->slice_interval = 0; return 0; }
+    const size_t bytes = (size_t)capacity * 9 * c->plane * sizeof(float);
+    HIP_TRY(hipMalloc(&c->slice_stage, bytes));
+    HIP_TRY(hipHostMalloc(&c->slice_host, bytes, hipHostMallocDefault));
+    if (!c->side) HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (!c->slice_ev) HIP_TRY(hipEventCreateWithFlags(&c->slice_ev, hipEventDisableTiming));
+    return 0;
+}
+
+extern "C" int fdtd_ctx_set_field(fdtd_ctx *c, int field, int level, const float *host)
+{
+    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(field_dev(c, field, level), host, c->N * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int fdtd_ctx_get_field(fdtd_ctx *c, int field, int level, float *host)
+{
+    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(host, field_dev(c, field, level), c->N * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fdtd_ctx_field_ptr(fdtd_ctx *c, int field, int level, float **dev)
+{
+    if (!c || !dev || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    *dev = field_dev(c, field, level);
+    return 0;
+}
+
+extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
+{
+    if (!c) return fail(FDTD_EINVAL, "null context");
+    const int n = c->n;
+    const int t = n % 3, s1 = (n + 2) % 3, s2 = (n + 1) % 3;
+    const int ho = c->hcur, hn = c->hcur ^ 1;
+
+    StepArgs a;
+    a.hxo = c->H[ho][0]; a.hyo = c->H[ho][1]; a.hzo = c->H[ho][2];
+    a.hxn = c->H[hn][0]; a.hyn = c->H[hn][1]; a.hzn = c->H[hn][2];
+    a.ex0 = c->E[s1][0]; a.ey0 = c->E[s1][1]; a.ez0 = c->E[s1][2];
+    a.ex1 = c->E[s2][0]; a.ey1 = c->E[s2][1]; a.ez1 = c->E[s2][2];
+    a.exT = c->E[t][0]; a.eyT = c->E[t][1]; a.ezT = c->E[t][2];
+    a.dx0 = c->D[s1][0]; a.dy0 = c->D[s1][1]; a.dz0 = c->D[s1][2];
+    a.dx1 = c->D[s2][0]; a.dy1 = c->D[s2][1]; a.dz1 = c->D[s2][2];
+    a.dxT = c->D[t][0]; a.dyT = c->D[t][1]; a.dzT = c->D[t][2];
+    a.ids = c->ids;
+    a.lut = c->lut;
+    a.nmat = c->nmat;
+    int idx = (c->ext.jz_index_mode == FDTD_JZ_HOLD3) ? (n / 3) * 3 : n;
+    const int lim = c->ext.jz_limit > 0 ? std::min<int>(c->ext.jz_limit, (int)c->jz.size()) : (int)c->jz.size();
+    a.jz = (idx >= 0 && idx < lim) ? c->jz[idx] : 0.0f;
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+    a.pitch = c->pitch;
+    a.plane = c->plane;
+    a.chx = c->dt / (c->mu0 * c->dx);
+    a.chy = c->dt / (c->mu0 * c->dy);
+    a.chz = c->dt / (c->mu0 * c->dz);
+    a.cdx = c->dt / c->dx;
+    a.cdy = c->dt / c->dy;
+    a.cdz = c->dt / c->dz;
+    a.kchunk = c->kchunk;
+    a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
+    a.xcd_remap = c->ext.xcd_remap;
+    const bool timed = c->tused < c->tcap;
+    if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
+    hipError_t e = launch_fused_step(a, c->ty, stream);
+    if (e != hipSuccess) return hip_fail(e, "fused_step");
+    if (timed) {
+        HIP_TRY(hipEventRecord(c->tev[2 * c->tused + 1], stream));
+        c->tused++;
+    }
+
+    MurArgs m;
+    m.exT = a.exT; m.eyT = a.eyT; m.ezT = a.ezT;
+    m.exS = a.ex0; m.eyS = a.ey0; m.ezS = a.ez0;
+    m.rpx0 = c->rp[0]; m.rpy0 = c->rp[1]; m.rpz0 = c->rp[2];
+    m.rpxEnd = c->rp[3]; m.rpyEnd = c->rp[4]; m.rpzEnd = c->rp[5];
+    m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pitch = c->pitch; m.plane = c->plane;
+    m.dt = c->dt; m.hx = c->dx; m.hy = c->dy; m.hz = c->dz; m.mu0 = c->mu0; m.eps0 = c->eps0;
+    mur_segments(c->nx, c->ny, c->nz, m.seg_start);
+    e = launch_mur(m, stream);
+    if (e != hipSuccess) return hip_fail(e, "mur_faces");
+
+    c->n = n + 1;
+    c->hcur = hn;
+
+    if (c->nprobe > 0 && c->probe_rows < c->probe_cap) {
+        ProbeArgs p;
+        for (int f = 0; f < 9; f++) p.field[f] = field_dev(c, f, 0);
+        p.probes = c->probes;
+        p.nprobe = c->nprobe;
+        p.pitch = c->pitch;
+        p.plane = c->plane;
+        p.out = c->probe_out;
+        p.row = c->probe_rows++;
+        e = launch_probes(p, stream);
+        if (e != hipSuccess) return hip_fail(e, "probe_cells");
+    }
+    if (c->slice_interval > 0 && c->n % c->slice_interval == 0 && c->slice_count < c->slice_cap) {
+        const size_t pl = (size_t)c->plane;
+        float *stage = c->slice_stage + (size_t)c->slice_count * 9 * pl;
+        for (int f = 0; f < 9; f++)
+            HIP_TRY(hipMemcpyAsync(stage + f * pl, field_dev(c, f, 0) + (size_t)c->slice_k * pl, pl * sizeof(float),
+                                   hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipEventRecord(c->slice_ev, stream));
+        HIP_TRY(hipStreamWaitEvent(c->side, c->slice_ev, 0));
+        HIP_TRY(hipMemcpyAsync(c->slice_host + (size_t)c->slice_count * 9 * pl, stage, 9 * pl * sizeof(float),
+                               hipMemcpyDeviceToHost, c->side));
+        c->slice_steps.push_back(c->n);
+        c->slice_count++;
+    }
+    return 0;
+}
+
+extern "C" int fdtd_run(fdtd_ctx *c, int steps, hipStream_t stream)
+{
+    if (!c || steps < 0) return fail(FDTD_EINVAL, "bad run");
+    for (int s = 0; s < steps; s++) {
+        int rc = fdtd_step_fused(c, stream);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int fdtd_ctx_steps_done(fdtd_ctx *c) { return c ? c->n : FDTD_EINVAL; }
+
+extern "C" int fdtd_probe_read(fdtd_ctx *c, double *host, int max_rows)
+{
+    if (!c || (!host && max_rows > 0)) return fail(FDTD_EINVAL, "bad probe read");
+    HIP_TRY(hipDeviceSynchronize());
+    const int rows = std::min(max_rows, c->probe_rows);
+    if (rows > 0 && c->nprobe > 0)
+        HIP_TRY(hipMemcpy(host, c->probe_out, sizeof(double) * (size_t)rows * c->nprobe, hipMemcpyDeviceToHost));
+    return c->nprobe > 0 ? rows : 0;
+}
+
+extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_slices)
+{
+    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (c->side) HIP_TRY(hipStreamSynchronize(c->side));
+    const int cnt = std::min(max_slices, c->slice_count);
+    const size_t per = (size_t)9 * c->plane;
+    for (int s = 0; s < cnt; s++) {
+        if (steps) steps[s] = c->slice_steps[s];
+        if (data) memcpy(data + s * per, c->slice_host + s * per, per * sizeof(float));
+    }
+    return cnt;
+}
+
+extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int *kchunk, int *blocks)
+{
+    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (tile_x) *tile_x = 64;
+    if (tile_y) *tile_y = c->ty;
+    if (kchunk) *kchunk = c->kchunk;
+    if (blocks) *blocks = c->tiles_x * c->tiles_y * c->tiles_z;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity)
+{
+    if (!c || capacity < 0) return fail(FDTD_EINVAL, "bad timing capacity");
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto ev : c->tev) (void)hipEventDestroy(ev);
+    c->tev.assign(2 * (size_t)capacity, nullptr);
+    for (auto &ev : c->tev) HIP_TRY(hipEventCreate(&ev));
+    c->tcap = capacity;
+    c->tused = 0;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_kernel_time(fdtd_ctx *c, double *ms_total, int *launches)
+{
+    if (!c || !ms_total) return fail(FDTD_EINVAL, "null argument");
+    double acc = 0.0;
+    for (int t = 0; t < c->tused; t++) {
+        HIP_TRY(hipEventSynchronize(c->tev[2 * t + 1]));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->tev[2 * t], c->tev[2 * t + 1]));
+        acc += ms;
+    }
+    *ms_total = acc;
+    if (launches) *launches = c->tused;
+    return 0;
+}

@@ -1,0 +1,242 @@
+"""Seeded synthetic workloads of the FDTD path.
+
+Every random draw comes from an in-house splitmix64 stream with an explicit
+float mapping (53 high-quality bits -> [0,1)), so the same seed builds the
+same grids on any host.  Grid parameters are derived by the library's own
+initParamsWithPath from a parameter file in the reference's format
+(data/input_params) plus extension keys, and the source waveform by its
+setupSources (fdtd.cu:1027-1082), so the workloads exercise the same setup
+code as the CLI.
+
+Workloads (sizes as stated):
+  plain_small   64^3, 2 mm cells, Courant 0.99, vacuum plus a centred Debye
+                sphere of radius 12 (eps_inf 40, eps_s 55, tau 8 ps,
+                sigma 0.8 S/m), Ez point source at (16,32,32), 2.45 GHz,
+                500 steps, Ez probe at the centre.
+  plain_256     256^3, 8 nested seeded ellipsoids (semi-axes 20-110 cells,
+                centres 128 +- 10), random Debye materials, Jz sheet on the
+                plane x = 10, 900 MHz, 2000 steps, 3 probes, z-slice k = 128
+                every 100 steps.
+  large_phantom 320x320x880, 50 tissue ids from seeded ellipsoids and
+                z-cylinders, 10-cell Jz dipole at the phantom edge, 900 MHz,
+                5000 steps.
+"""
+import ctypes as C
+import dataclasses
+import os
+import tempfile
+
+import numpy as np
+
+from . import FdtdParams, lib
+
+C0 = 3.0e8
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014) with a 53-bit float mapping."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.state = seed & self.MASK
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def randint(self, lo, hi):
+        """Integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    shape: tuple            # (nx, ny, nz)
+    params: FdtdParams
+    ids: np.ndarray         # uint8 (nz, ny, nx)
+    table: np.ndarray       # float32 (nmat, 4): sigma, eps_s, eps_inf, tau
+    sources: np.ndarray     # int32 (nsrc, 3), 0-based
+    jz: np.ndarray          # float32 waveform, one entry per step
+    probes: np.ndarray      # int32 (nprobe, 4): field id, i, j, k
+    steps: int
+    slice_k: int = -1
+    slice_every: int = 0
+    seed: int = 0
+    freq: float = 0.0
+
+    @property
+    def cells(self):
+        nx, ny, nz = self.shape
+        return nx * ny * nz
+
+    def material_arrays(self):
+        """The reference's four per-voxel arrays (fdtd.h:50-51)."""
+        t = self.table[self.ids.astype(np.int64)]
+        return t[..., 0].copy(), t[..., 1].copy(), t[..., 2].copy(), t[..., 3].copy()
+
+    def rp_planes(self):
+        """Mur permittivity planes of the vacuum default material: 1.0."""
+        nx, ny, nz = self.shape
+        return [np.ones(n, np.float32) for n in (2 * ny * nz, nx * 2 * nz, nx * ny * 2) * 2]
+
+
+def params_text(shape, steps, freq, cell, courant, pulse_width=0.5, pmf=1.0, sources=((1, 1, 1),)):
+    """A parameter file in the reference's positional format
+    (data/input_params) followed by extension keys."""
+    nx, ny, nz = shape
+    lines = [
+        f"nx_ny_nz: {nx} {ny} {nz}", f"t_max: {steps}", "nf: 1", "env_set_dir: data/pgmdata",
+        'env_file_prefix: "ver"', "output_dir: output", 'output_format: "txt"', "impulse_resp_flag: 0",
+        "pec_flag: 0", "read_env_flag: 0", "output_flag: 1", "bzip2_flag: 0", "output_start: 0 0 0",
+        "output_finish: 0 0 0", "source_type:  soft", "elements_per_wavelength: 100",
+        f"wave_frequency: {freq:.9e}", f"pulse_width: {pulse_width}", f"pulse_modulation_frequency: {pmf}",
+        f"number_of_excitation_sources: {len(sources)}",
+    ]
+    lines += [f"source_location: {s[0] + 1} {s[1] + 1} {s[2] + 1}" for s in sources]
+    lines += ["pulse_type: 2", "fsigma: 0", "feps_s: 1", "feps_inf: 1", "ftau_d: 0",
+              f"courant_factor: {courant}", f"cell_size: {cell:.9e}"]
+    return "\n".join(lines) + "\n"
+
+
+def derive(shape, steps, freq, cell, courant, pulse_width=0.5, pmf=1.0):
+    """(FdtdParams copy, jz waveform) from the library's own
+    initParamsWithPath + setupSources."""
+    L = lib()
+    with tempfile.NamedTemporaryFile("w", suffix=".params", delete=False) as f:
+        f.write(params_text(shape, steps, freq, cell, courant, pulse_width, pmf))
+        path = f.name
+    try:
+        pp = L.initParamsWithPath(path.encode())
+        L.setupSources(pp)
+        p = pp.contents
+        jz = np.ctypeslib.as_array(p.jz, shape=(p.jzCount,)).astype(np.float32).copy()
+        out = FdtdParams()
+        C.memmove(C.byref(out), C.byref(p), C.sizeof(FdtdParams))
+        out.inputPath = None
+        out.outputPath = None
+        out.sources = None
+        out.jz = None
+        out.sourcesCount = 0
+        out.jzCount = 0
+        L.deallocParams(pp)
+    finally:
+        os.unlink(path)
+    return out, jz
+
+
+def debye_table(rng, n):
+    """n random Debye materials: eps_inf U[4,45], eps_s U[5,80] (the pair
+    sorted so eps_s >= eps_inf), tau U[5,50] ps, sigma U[0.02,2.5] S/m."""
+    rows = []
+    for _ in range(n):
+        ei = rng.uniform(4.0, 45.0)
+        es = rng.uniform(5.0, 80.0)
+        ei, es = min(ei, es), max(ei, es)
+        tau = rng.uniform(5e-12, 50e-12)
+        sg = rng.uniform(0.02, 2.5)
+        rows.append((sg, es, ei, tau))
+    return np.asarray(rows, np.float32)
+
+
+def _paint_ellipsoid(ids, c, a, value):
+    nz, ny, nx = ids.shape
+    lo = [max(0, int(np.floor(c[d] - a[d]))) for d in range(3)]
+    hi = [min(n, int(np.ceil(c[d] + a[d])) + 1) for d, n in zip(range(3), (nx, ny, nz))]
+    if any(h <= l for l, h in zip(lo, hi)):
+        return
+    x = (np.arange(lo[0], hi[0]) - c[0]) / a[0]
+    y = (np.arange(lo[1], hi[1]) - c[1]) / a[1]
+    z = (np.arange(lo[2], hi[2]) - c[2]) / a[2]
+    m = (z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2) <= 1.0
+    sub = ids[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+    sub[m] = value
+
+
+def _paint_zcylinder(ids, c, r, z0, z1, value):
+    nz, ny, nx = ids.shape
+    lo = [max(0, int(np.floor(c[0] - r))), max(0, int(np.floor(c[1] - r))), max(0, z0)]
+    hi = [min(nx, int(np.ceil(c[0] + r)) + 1), min(ny, int(np.ceil(c[1] + r)) + 1), min(nz, z1)]
+    if any(h <= l for l, h in zip(lo, hi)):
+        return
+    x = np.arange(lo[0], hi[0]) - c[0]
+    y = np.arange(lo[1], hi[1]) - c[1]
+    m2 = (y[:, None] ** 2 + x[None, :] ** 2) <= r * r
+    sub = ids[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+    sub[:, m2] = value
+
+
+def plain_small(steps=500):
+    """64^3 vacuum with a Debye sphere; CPU-runnable reference case."""
+    shape = (64, 64, 64)
+    params, jz = derive(shape, steps, 2.45e9, 2.0e-3, 0.99)
+    ids = np.zeros(shape[::-1], np.uint8)
+    k, j, i = np.meshgrid(np.arange(64), np.arange(64), np.arange(64), indexing="ij")
+    ids[(i - 32) ** 2 + (j - 32) ** 2 + (k - 32) ** 2 <= 144] = 1
+    table = np.asarray([(0.0, 1.0, 1.0, 0.0), (0.8, 55.0, 40.0, 8e-12)], np.float32)
+    return Workload("plain_small", shape, params, ids, table, np.asarray([[16, 32, 32]], np.int32),
+                    jz[:max(steps, 1)].copy(), np.asarray([[8, 32, 32, 32]], np.int32), steps, freq=2.45e9)
+
+
+def plain_256(steps=2000, seed=12347, n=256):
+    """n^3 (256 by default) with 8 nested seeded ellipsoids and a Jz sheet on x = 10."""
+    shape = (n, n, n)
+    params, jz = derive(shape, steps, 9.0e8, 2.0e-3, 0.99)
+    rng = SplitMix64(seed)
+    s = n / 256.0
+    shapes = []
+    for _ in range(8):
+        a = [rng.uniform(20.0, 110.0) * s for _ in range(3)]
+        c = [n / 2 + rng.uniform(-10.0, 10.0) * s for _ in range(3)]
+        shapes.append((a[0] * a[1] * a[2], c, a))
+    shapes.sort(key=lambda t: -t[0])  # largest first, so later ids nest inside
+    table = np.concatenate([np.asarray([(0.0, 1.0, 1.0, 0.0)], np.float32), debye_table(rng, 8)])
+    ids = np.zeros(shape[::-1], np.uint8)
+    for m, (_, c, a) in enumerate(shapes, start=1):
+        _paint_ellipsoid(ids, c, a, m)
+    x = max(2, int(10 * s))
+    jj, kk = np.meshgrid(np.arange(2, n - 2), np.arange(2, n - 2), indexing="ij")
+    src = np.stack([np.full(jj.size, x), jj.ravel(), kk.ravel()], 1).astype(np.int32)
+    h = n // 2
+    probes = np.asarray([[8, h, h, h], [8, n // 4, h, h], [8, 3 * n // 4, h, h]], np.int32)
+    return Workload("plain_256", shape, params, ids, table, src, jz[:max(steps, 1)].copy(), probes, steps,
+                    slice_k=h, slice_every=100, seed=seed, freq=9.0e8)
+
+
+def large_phantom(steps=5000, seed=12348, shape=(320, 320, 880)):
+    """320x320x880 voxel phantom of 50 tissue ids (ellipsoids and z-cylinders)."""
+    nx, ny, nz = shape
+    params, jz = derive(shape, steps, 9.0e8, 2.0e-3, 0.99)
+    rng = SplitMix64(seed)
+    table = np.concatenate([np.asarray([(0.0, 1.0, 1.0, 0.0)], np.float32), debye_table(rng, 50)])
+    ids = np.zeros((nz, ny, nx), np.uint8)
+    # body: a large ellipsoid, then 49 seeded organs/limbs inside its box
+    body_c = (nx / 2, ny / 2, nz / 2)
+    body_a = (0.40 * nx, 0.30 * ny, 0.45 * nz)
+    _paint_ellipsoid(ids, body_c, body_a, 1)
+    for m in range(2, 51):
+        cx = body_c[0] + rng.uniform(-0.7, 0.7) * body_a[0]
+        cy = body_c[1] + rng.uniform(-0.7, 0.7) * body_a[1]
+        cz = body_c[2] + rng.uniform(-0.8, 0.8) * body_a[2]
+        if m % 2 == 0:
+            a = (rng.uniform(8, 45), rng.uniform(8, 40), rng.uniform(10, 90))
+            _paint_ellipsoid(ids, (cx, cy, cz), a, m)
+        else:
+            r = rng.uniform(5, 25)
+            half = rng.uniform(20, 150)
+            _paint_zcylinder(ids, (cx, cy), r, int(cz - half), int(cz + half), m)
+    # dipole: 10 cells of Jz along z just outside the body surface on -x
+    xs = max(2, int(body_c[0] - body_a[0]) - 4)
+    zc = nz // 2
+    src = np.asarray([[xs, ny // 2, z] for z in range(zc - 5, zc + 5)], np.int32)
+    probes = np.asarray([[8, nx // 2, ny // 2, nz // 2], [8, xs + 8, ny // 2, zc], [6, nx // 2, ny // 2, nz // 4]],
+                        np.int32)
+    return Workload("large_phantom", shape, params, ids, table, src, jz[:max(steps, 1)].copy(), probes, steps,
+                    slice_k=nz // 2, slice_every=500, seed=seed, freq=9.0e8)

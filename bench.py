@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--workload", default="plain_256", choices=["plain_256", "large_phantom", "plain_small"])
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run one device-to-device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
                     help="measured FETCH_SIZE and WRITE_SIZE per fused launch (KiB, already scaled)")
     return ap.parse_args()
@@ -75,6 +77,13 @@ def main():
         sys.exit(2)
     import fdtd_amd as F
     from fdtd_amd import hip
+
+    if a.copy_bytes:
+        src, dst = hip.DeviceArray(a.copy_bytes // 4), hip.DeviceArray(a.copy_bytes // 4)
+        hip.ok(hip.rt().hipMemcpy(dst.ptr, src.ptr, a.copy_bytes, hip.D2D), "calibration copy")
+        hip.sync()
+        src.free()
+        dst.free()
 
     total = a.warmup + a.steps
     w = build_workload(a.workload, total)

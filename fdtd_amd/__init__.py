@@ -165,7 +165,7 @@ def make_params(shape, dt, dx, dy=None, dz=None, mu0=None, eps0=None):
 class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
-    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=1):
+    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))

@@ -67,7 +67,9 @@ struct fdtd_ctx {
     FdtdExt ext;
     int ty, kchunk, tiles_x, tiles_y, tiles_z;
 
+    long long guard = 0;                 // elements before/after the field arrays and the ids
     float *blob = nullptr;               // all field arrays
+    unsigned char *ids_base = nullptr;   // allocation holding ids with its guards
     float *E[3][3], *D[3][3], *H[2][3];  // [slot][component], H[buffer][component]
     int hcur = 0;                        // H buffer holding the newest H
     unsigned char *ids = nullptr;
@@ -133,7 +135,7 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->jz_limit = 0;
     ext->tile_y = 0;
     ext->kchunk = 0;
-    ext->xcd_remap = 1;
+    ext->xcd_remap = 0;
     return 0;
 }
 
@@ -143,14 +145,16 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     *out = nullptr;
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
     const long long N = (long long)p->nx * p->ny * p->nz;
-    // The fused kernel addresses each field array with 32-bit byte offsets.
-    if (N >= (1LL << 30)) return fail(FDTD_ERANGE, "grid of %lld cells: at most 2^30 - 1 per context", N);
+    // The fused kernel addresses each field array with 32-bit byte offsets
+    // that span the array and its guards.
+    if (N + 2 * fused_guard(p->nx, (long long)p->nx * p->ny) >= (1LL << 30))
+        return fail(FDTD_ERANGE, "grid of %lld cells: too large for one context", N);
     FdtdExt e;
     fdtd_ext_defaults(&e);
     if (ext) e = *ext;
     if (e.debye_mode != FDTD_DEBYE_REF && e.debye_mode != FDTD_DEBYE_FIXED) return fail(FDTD_EINVAL, "debye_mode %d", e.debye_mode);
     if (e.jz_index_mode != FDTD_JZ_STEP && e.jz_index_mode != FDTD_JZ_HOLD3) return fail(FDTD_EINVAL, "jz_index_mode %d", e.jz_index_mode);
-    if (e.tile_y != 0 && e.tile_y != 1 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
+    if (e.tile_y != 0 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
     if (e.kchunk < 0 || e.jz_limit < 0) return fail(FDTD_EINVAL, "negative kchunk/jz_limit");
 
     fdtd_ctx *c = new fdtd_ctx();
@@ -160,32 +164,30 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->plane = (long long)p->nx * p->ny;
     c->dt = p->dt; c->dx = p->dx; c->dy = p->dy; c->dz = p->dz; c->mu0 = p->mu0; c->eps0 = p->eps0;
     c->ext = e;
+    // Defaults from launch-shape sweeps on MI355X (256^3 and 320x320x880
+    // Debye phantoms): 64x4 tiles marching 8 planes, no XCD remap.
     c->ty = e.tile_y ? e.tile_y : 4;
     c->tiles_x = (c->nx + 63) / 64;
     c->tiles_y = (c->ny + c->ty - 1) / c->ty;
-    if (e.kchunk) {
-        c->kchunk = std::min(e.kchunk, c->nz);
-    } else {
-        const long long target = 4096;
-        const long long txy = (long long)c->tiles_x * c->tiles_y;
-        long long kc = (c->nz * txy + target - 1) / target;
-        c->kchunk = (int)std::max(4LL, std::min<long long>(kc, c->nz));
-    }
+    c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : std::min(8, c->nz);
     c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H
-    hipError_t he = hipMalloc(&c->blob, nf * c->N * sizeof(float));
+    c->guard = fused_guard(c->pitch, c->plane);
+    const size_t blob_elems = nf * c->N + 2 * (size_t)c->guard;
+    hipError_t he = hipMalloc(&c->blob, blob_elems * sizeof(float));
     if (he != hipSuccess) { delete c; return hip_fail(he, "hipMalloc(fields)"); }
-    float *q = c->blob;
+    float *q = c->blob + c->guard;
     for (int s = 0; s < 3; s++)
         for (int k = 0; k < 3; k++) { c->E[s][k] = q; q += c->N; }
     for (int s = 0; s < 3; s++)
         for (int k = 0; k < 3; k++) { c->D[s][k] = q; q += c->N; }
     for (int s = 0; s < 2; s++)
         for (int k = 0; k < 3; k++) { c->H[s][k] = q; q += c->N; }
-    he = hipMemset(c->blob, 0, nf * c->N * sizeof(float));
-    if (he == hipSuccess) he = hipMalloc(&c->ids, c->N);
-    if (he == hipSuccess) he = hipMemset(c->ids, 0, c->N);
+    he = hipMemset(c->blob, 0, blob_elems * sizeof(float));
+    if (he == hipSuccess) he = hipMalloc(&c->ids_base, c->N + 2 * (size_t)c->guard);
+    if (he == hipSuccess) he = hipMemset(c->ids_base, 0, c->N + 2 * (size_t)c->guard);
+    if (he == hipSuccess) c->ids = c->ids_base + c->guard;
     if (he == hipSuccess) he = hipMalloc(&c->lut, sizeof(MatCoef) * kMaxMaterials);
     const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
                            (size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2};
@@ -219,7 +221,7 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
     if (!c) return 0;
     if (c->side) (void)hipStreamSynchronize(c->side);
     (void)hipFree(c->blob);
-    (void)hipFree(c->ids);
+    (void)hipFree(c->ids_base);
     (void)hipFree(c->lut);
     for (int r = 0; r < 6; r++) (void)hipFree(c->rp[r]);
     (void)hipFree(c->probes);
@@ -406,6 +408,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.kchunk = c->kchunk;
     a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
     a.xcd_remap = c->ext.xcd_remap;
+    a.guard = (unsigned)c->guard;
     const bool timed = c->tused < c->tcap;
     if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
     hipError_t e = launch_fused_step(a, c->ty, stream);

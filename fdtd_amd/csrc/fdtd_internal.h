@@ -48,7 +48,13 @@ struct StepArgs {
     int kchunk;                     // z planes per block
     int tiles_x, tiles_y, tiles_z;  // logical tile grid
     int xcd_remap;                  // 1: map logical tiles to XCDs in contiguous runs
+    unsigned guard;                 // addressable elements before and after every field array (and ids)
 };
+
+// Guard (in elements) the fused kernel needs around every field array and
+// the id array so that its halo loads need no bounds checks: one plane, ten
+// rows and 256 elements (tiles are at most 8 rows high and 64 wide).
+inline long long fused_guard(int pitch, long long plane) { return plane + 10LL * pitch + 256; }
 
 // Arguments of the 6-face first-order Mur kernel (fdtd_calculations.cu:216-428).
 struct MurArgs {

@@ -54,17 +54,38 @@ __device__ __forceinline__ float ldg(const float *p, uint32_t boff)
 
 __device__ __forceinline__ void stg(float *p, uint32_t boff, float v) { *(float *)((char *)p + boff) = v; }
 
+// Streaming (nontemporal) forms for data each launch touches once: the
+// ring levels n and n-1 of D, level n-1 of E, the ids and every store.
+__device__ __forceinline__ float ldg_nt(const float *p, uint32_t boff)
+{
+    return __builtin_nontemporal_load((const float *)((const char *)p + boff));
+}
+
+__device__ __forceinline__ void stg_nt(float *p, uint32_t boff, float v)
+{
+    __builtin_nontemporal_store(v, (float *)((char *)p + boff));
+}
+
 // All per-lane addresses are a 64-bit array base (uniform, in SGPRs) plus a
 // 32-bit byte offset shared by every array indexed by the same cell, so the
-// loads use the scalar-base form with one offset register.  The host keeps
-// every array below 4 GiB.
+// loads use the scalar-base form with one offset register.  Offsets are
+// measured from `guard` elements before each array's origin: every array
+// (and the id array) has at least `guard` addressable elements before and
+// after it, so the halo loads at i-1, i+64, j-1, j+TY, k-1 and k+1 need no
+// bounds checks (their values are discarded outside the update ranges).
+//
+// Roles inside a block (wave-uniform): wave 0 also loads and updates the
+// -y halo row, wave TY-1 loads the +y E row, and wave 1 gathers the -x/+x
+// E columns and updates the -x halo H column (lane l: side l/32, row l%32).
+// Each plane's global loads are issued one plane ahead of their use.
 template <int TY>
 __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
 {
+    static_assert(TY >= 2 && TY <= 8, "tile rows");
     __shared__ float sE[2][3][TY + 2][TX + 2];
     __shared__ float sH[3][TY + 1][TX + 1];
-    __shared__ float lA[kMaxMaterials], lC[kMaxMaterials], lD[kMaxMaterials];
-    __shared__ double lG1[kMaxMaterials], lG2[kMaxMaterials], lG3[kMaxMaterials];
+    __shared__ float sJunk[2][TX];
+    extern __shared__ double sLut[];  // g1[nmat] g2[nmat] g3[nmat] then float a[nmat] c[nmat] d[nmat]
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
@@ -79,187 +100,196 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     const int k1 = min(k0 + A.kchunk, nz);
     const int i = i0 + tx, j = j0 + ty;
     const bool own = (i < nx) && (j < ny);
-    const uint32_t syb = (uint32_t)A.pitch * 4u, szb = (uint32_t)A.plane * 4u;
-    const uint32_t colb = (uint32_t)j * syb + (uint32_t)i * 4u;  // byte offset of (i, j) within a plane
 
-    for (int m = tid; m < A.nmat; m += TX * TY) {
+    const int nmat = A.nmat;
+    double *lG1 = sLut, *lG2 = sLut + nmat, *lG3 = sLut + 2 * nmat;
+    float *lA = (float *)(sLut + 3 * nmat), *lC = lA + nmat, *lD = lA + 2 * nmat;
+    for (int m = tid; m < nmat; m += TX * TY) {
         const MatCoef mc = A.lut[m];
         lA[m] = mc.a; lC[m] = mc.c; lD[m] = mc.d;
         lG1[m] = mc.g1; lG2[m] = mc.g2; lG3[m] = mc.g3;
     }
 
-    auto ld = [&](const float *p, int ii, int jj, int kk) -> float {
-        return (ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz)
-                   ? ldg(p, (uint32_t)kk * szb + (uint32_t)jj * syb + (uint32_t)ii * 4u)
-                   : 0.0f;
-    };
+    // Origin-shifted bases (guard elements before each array's origin).
+    const uint32_t gb = A.guard * 4u;
+    auto sh = [&](const float *p) { return (const float *)((const char *)p - gb); };
+    const float *hxo = sh(A.hxo), *hyo = sh(A.hyo), *hzo = sh(A.hzo);
+    const float *ex0 = sh(A.ex0), *ey0 = sh(A.ey0), *ez0 = sh(A.ez0);
+    const float *ex1 = sh(A.ex1), *ey1 = sh(A.ey1), *ez1 = sh(A.ez1);
+    const float *dx0 = sh(A.dx0), *dy0 = sh(A.dy0), *dz0 = sh(A.dz0);
+    const float *dx1 = sh(A.dx1), *dy1 = sh(A.dy1), *dz1 = sh(A.dz1);
+    float *hxn = (float *)sh(A.hxn), *hyn = (float *)sh(A.hyn), *hzn = (float *)sh(A.hzn);
+    float *exT = (float *)sh(A.exT), *eyT = (float *)sh(A.eyT), *ezT = (float *)sh(A.ezT);
+    float *dxT = (float *)sh(A.dxT), *dyT = (float *)sh(A.dyT), *dzT = (float *)sh(A.dzT);
+    const uint8_t *ids = A.ids - A.guard;
 
-    // Stage E^n of plane k (own cell plus the halo cells the H stencils of
-    // this tile and of its -x/-y halo need) into buffer `buf`.
-    auto load_plane = [&](int buf, int k) {
-        const bool kin = (k >= 0 && k < nz);
-        const uint32_t ob = (uint32_t)k * szb + colb;
-        const bool ok = own && kin;
-        sE[buf][0][ty + 1][tx + 1] = ok ? ldg(A.ex0, ob) : 0.0f;
-        sE[buf][1][ty + 1][tx + 1] = ok ? ldg(A.ey0, ob) : 0.0f;
-        sE[buf][2][ty + 1][tx + 1] = ok ? ldg(A.ez0, ob) : 0.0f;
-        if (ty == 0) {
-            sE[buf][0][0][tx + 1] = ld(A.ex0, i, j0 - 1, k);
-            sE[buf][1][0][tx + 1] = ld(A.ey0, i, j0 - 1, k);
-            sE[buf][2][0][tx + 1] = ld(A.ez0, i, j0 - 1, k);
-        }
-        if (ty == TY - 1) {
-            sE[buf][0][TY + 1][tx + 1] = ld(A.ex0, i, j0 + TY, k);
-            sE[buf][2][TY + 1][tx + 1] = ld(A.ez0, i, j0 + TY, k);
-        }
-        if (tx == 0) {
-            sE[buf][0][ty + 1][0] = ld(A.ex0, i0 - 1, j, k);
-            sE[buf][1][ty + 1][0] = ld(A.ey0, i0 - 1, j, k);
-            sE[buf][2][ty + 1][0] = ld(A.ez0, i0 - 1, j, k);
-            if (ty == TY - 1) sE[buf][0][TY + 1][0] = ld(A.ex0, i0 - 1, j0 + TY, k);
-        }
-        if (tx == TX - 1) {
-            sE[buf][1][ty + 1][TX + 1] = ld(A.ey0, i0 + TX, j, k);
-            sE[buf][2][ty + 1][TX + 1] = ld(A.ez0, i0 + TX, j, k);
-            if (ty == 0) sE[buf][1][0][TX + 1] = ld(A.ey0, i0 + TX, j0 - 1, k);
-        }
-    };
+    const uint32_t syb = (uint32_t)A.pitch * 4u, szb = (uint32_t)A.plane * 4u;
+    // plane-relative byte offsets (wrap-around arithmetic handles j0-1 = -1 etc.)
+    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)i * 4u;
+    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)i * 4u;
+    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)i * 4u;
+    const int side = tx >> 5;
+    const int cr = min(tx & 31, TY + 1);             // LDS row of the gathered column cell
+    const int cl = side ? TX + 1 : 0;                 // LDS column of the gathered cell
+    const int ci = side ? i0 + TX : i0 - 1;
+    const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * 4u;
+    const bool w_rlo = (ty == 0), w_rhi = (ty == TY - 1), w_col = (ty == 1);
 
     const float chx = A.chx, chy = A.chy, chz = A.chz;
     const float cdx = A.cdx, cdy = A.cdy, cdz = A.cdz;
 
+    // E^n staging registers of one plane: own cell, -y row (wave 0), +y row
+    // (wave TY-1, Ex and Ez), gathered columns (wave 1).
+    float pe[3], pr[3], pq[2], pc[3];
+    auto fetch_e = [&](int k) {
+        const uint32_t kb = (uint32_t)k * szb;
+        pe[0] = ldg(ex0, kb + c_own); pe[1] = ldg(ey0, kb + c_own); pe[2] = ldg(ez0, kb + c_own);
+        if (w_rlo) { pr[0] = ldg(ex0, kb + c_rlo); pr[1] = ldg(ey0, kb + c_rlo); pr[2] = ldg(ez0, kb + c_rlo); }
+        if (w_rhi) { pq[0] = ldg(ex0, kb + c_rhi); pq[1] = ldg(ez0, kb + c_rhi); }
+        if (w_col) { pc[0] = ldg(ex0, kb + c_col); pc[1] = ldg(ey0, kb + c_col); pc[2] = ldg(ez0, kb + c_col); }
+    };
+    auto stage_e = [&](int buf) {
+        sE[buf][0][ty + 1][tx + 1] = pe[0]; sE[buf][1][ty + 1][tx + 1] = pe[1]; sE[buf][2][ty + 1][tx + 1] = pe[2];
+        if (w_rlo) { sE[buf][0][0][tx + 1] = pr[0]; sE[buf][1][0][tx + 1] = pr[1]; sE[buf][2][0][tx + 1] = pr[2]; }
+        if (w_rhi) { sE[buf][0][TY + 1][tx + 1] = pq[0]; sE[buf][2][TY + 1][tx + 1] = pq[1]; }
+        if (w_col) { sE[buf][0][cr][cl] = pc[0]; sE[buf][1][cr][cl] = pc[1]; sE[buf][2][cr][cl] = pc[2]; }
+    };
+
+    // Own-cell operands of one plane.
+    struct Own {
+        float hxo, hyo, hzo, dxn, dyn, dzn, dxm, dym, dzm, exm, eym, ezm, hh0, hh1;
+        uint32_t id;
+    };
+    auto fetch_own = [&](int k, Own &o) {
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        o.hxo = ldg(hxo, ob); o.hyo = ldg(hyo, ob); o.hzo = ldg(hzo, ob);
+        o.dxn = ldg_nt(dx0, ob); o.dyn = ldg_nt(dy0, ob); o.dzn = ldg_nt(dz0, ob);
+        o.dxm = ldg_nt(dx1, ob); o.dym = ldg_nt(dy1, ob); o.dzm = ldg_nt(dz1, ob);
+        o.exm = ldg_nt(ex1, ob); o.eym = ldg_nt(ey1, ob); o.ezm = ldg_nt(ez1, ob);
+        o.id = __builtin_nontemporal_load(ids + (ob >> 2));
+        // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TX, row cr): Hy, Hz
+        o.hh0 = 0.f; o.hh1 = 0.f;
+        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = ldg(hxo, oh); o.hh1 = ldg(hzo, oh); }
+        if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = ldg(hyo, oh); o.hh1 = ldg(hzo, oh); }
+    };
+
     float hxm = 0.0f, hym = 0.0f;  // own-column Hx, Hy of plane k-1 (time level n+1/2)
     int cur = 0;
     if (k0 > 0) {
-        load_plane(0, k0 - 1);
-        load_plane(1, k0);
+        fetch_e(k0 - 1);
+        stage_e(0);
+        fetch_e(k0);
+        stage_e(1);
         __syncthreads();
         const int k = k0 - 1;
-        if (own) {
-            const uint32_t ob = (uint32_t)k * szb + colb;
-            float hx = ldg(A.hxo, ob), hy = ldg(A.hyo, ob);
-            const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
-            const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
-            if (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1)
-                hx = hx - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
-            if (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1)
-                hy = hy - chz * (exk1 - exk) + chx * (sE[0][2][ty + 1][tx + 2] - ezk);
-            hxm = hx;
-            hym = hy;
-        }
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        const float hx0 = ldg(hxo, ob), hy0 = ldg(hyo, ob);
+        const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
+        const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
+        const float ux = hx0 - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
+        const float uy = hy0 - chz * (exk1 - exk) + chx * (sE[0][2][ty + 1][tx + 2] - ezk);
+        hxm = (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1) ? ux : hx0;
+        hym = (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1) ? uy : hy0;
         cur = 1;
         __syncthreads();
     } else {
-        load_plane(0, k0);
+        fetch_e(k0);
+        stage_e(0);
     }
+    Own cu, nx_;
+    fetch_own(k0, cu);
+    fetch_e(k0 + 1);
 
-    const bool row_halo = (ty == 0) && (j0 >= 1) && (i < nx);
-    const bool col_halo = (tx == 0) && (i0 >= 1) && (j < ny);
-    const uint32_t rowb = (uint32_t)(j0 - 1) * syb + (uint32_t)i * 4u;   // (i, j0-1)
-    const uint32_t colhb = (uint32_t)j * syb + (uint32_t)(i0 - 1) * 4u;  // (i0-1, j)
+    // Range masks that do not depend on k.
+    const bool hx_ij = (i >= 1 && i < nx - 1 && j < ny - 1);
+    const bool hy_ij = (i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool hz_ij = (i < nx - 1 && j < ny - 1);
+    const bool dx_ij = (i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool dy_ij = (i >= 1 && i < nx - 1 && j < ny - 1);
+    const bool dz_ij = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool e_i = (i >= 1);
+    // halo row (i, j0-1) and halo column (i0-1, j0-1+cr) masks
+    const int jh = j0 - 1;
+    const bool rx_ij = (jh >= 0) && (i >= 1 && i < nx - 1 && jh < ny - 1);
+    const bool rz_ij = (jh >= 0) && (i < nx - 1 && jh < ny - 1);
+    const int ccr = min(max(cr, 1), TY);              // clamped LDS row for the column compute
+    const int jc = j0 - 1 + ccr;
+    const bool col_ok = (side == 0) && (cr >= 1) && (cr <= TY) && (i0 >= 1);
+    const bool cy_ij = col_ok && (i0 - 1 < nx - 1) && (jc >= 1 && jc < ny - 1);
+    const bool cz_ij = col_ok && (i0 - 1 < nx - 1) && (jc < ny - 1);
+    float *col_dst_y = col_ok ? &sH[1][ccr][0] : &sJunk[0][tx];
+    float *col_dst_z = col_ok ? &sH[2][ccr][0] : &sJunk[1][tx];
 
+    const float jzv = A.jz;
     for (int k = k0; k < k1; ++k) {
         const int nxt = cur ^ 1;
-        const uint32_t ob = (uint32_t)k * szb + colb;
-
-        // Own-cell operands of this plane.
-        float hxo = 0.f, hyo = 0.f, hzo = 0.f, dxn = 0.f, dyn = 0.f, dzn = 0.f, dxm = 0.f, dym = 0.f, dzm = 0.f;
-        float exm = 0.f, eym = 0.f, ezm = 0.f;
-        int id = 0;
-        if (own) {
-            hxo = ldg(A.hxo, ob); hyo = ldg(A.hyo, ob); hzo = ldg(A.hzo, ob);
-            dxn = ldg(A.dx0, ob); dyn = ldg(A.dy0, ob); dzn = ldg(A.dz0, ob);
-            dxm = ldg(A.dx1, ob); dym = ldg(A.dy1, ob); dzm = ldg(A.dz1, ob);
-            exm = ldg(A.ex1, ob); eym = ldg(A.ey1, ob); ezm = ldg(A.ez1, ob);
-            id = A.ids[ob >> 2];
+        stage_e(nxt);                 // E^n of plane k+1
+        const bool more = (k + 1 < k1);
+        if (more) {
+            fetch_own(k + 1, nx_);
+            fetch_e(k + 2);
         }
-        // Halo H^{n-1/2}: row j0-1 (Hx, Hz) and column i0-1 (Hy, Hz).
-        float hrx = 0.f, hrz = 0.f, hcy = 0.f, hcz = 0.f;
-        if (row_halo) {
-            const uint32_t oh = (uint32_t)k * szb + rowb;
-            hrx = ldg(A.hxo, oh); hrz = ldg(A.hzo, oh);
-        }
-        if (col_halo) {
-            const uint32_t oh = (uint32_t)k * szb + colhb;
-            hcy = ldg(A.hyo, oh); hcz = ldg(A.hzo, oh);
-        }
-
-        load_plane(nxt, k + 1);
         __syncthreads();  // planes k (cur) and k+1 (nxt) staged
 
+        const bool kh = (k < nz - 1), kz = (k >= 1 && k < nz - 1);
+        // own H^{n+1/2}
         const float exk = sE[cur][0][ty + 1][tx + 1], eyk = sE[cur][1][ty + 1][tx + 1], ezk = sE[cur][2][ty + 1][tx + 1];
-        float hx = hxo, hy = hyo, hz = hzo;
-        if (own) {
-            const float exk1 = sE[nxt][0][ty + 1][tx + 1], eyk1 = sE[nxt][1][ty + 1][tx + 1];
-            if (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1)
-                hx = hx - chy * (sE[cur][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
-            if (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1)
-                hy = hy - chz * (exk1 - exk) + chx * (sE[cur][2][ty + 1][tx + 2] - ezk);
-            if (i < nx - 1 && j < ny - 1 && k >= 1 && k < nz - 1)
-                hz = hz - chx * (sE[cur][1][ty + 1][tx + 2] - eyk) + chy * (sE[cur][0][ty + 2][tx + 1] - exk);
-        }
+        const float exk1 = sE[nxt][0][ty + 1][tx + 1], eyk1 = sE[nxt][1][ty + 1][tx + 1];
+        const float ux = cu.hxo - chy * (sE[cur][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
+        const float uy = cu.hyo - chz * (exk1 - exk) + chx * (sE[cur][2][ty + 1][tx + 2] - ezk);
+        const float uz = cu.hzo - chx * (sE[cur][1][ty + 1][tx + 2] - eyk) + chy * (sE[cur][0][ty + 2][tx + 1] - exk);
+        const float hx = (hx_ij && kh) ? ux : cu.hxo;
+        const float hy = (hy_ij && kh) ? uy : cu.hyo;
+        const float hz = (hz_ij && kz) ? uz : cu.hzo;
         sH[0][ty + 1][tx + 1] = hx;
         sH[1][ty + 1][tx + 1] = hy;
         sH[2][ty + 1][tx + 1] = hz;
-        if (ty == 0) {
-            const int jh = j0 - 1;
-            if (row_halo) {
-                const float ezh = sE[cur][2][0][tx + 1], eyh = sE[cur][1][0][tx + 1], exh = sE[cur][0][0][tx + 1];
-                if (i >= 1 && i < nx - 1 && jh < ny - 1 && k < nz - 1)
-                    hrx = hrx - chy * (sE[cur][2][1][tx + 1] - ezh) + chy * (sE[nxt][1][0][tx + 1] - eyh);
-                if (i < nx - 1 && jh < ny - 1 && k >= 1 && k < nz - 1)
-                    hrz = hrz - chx * (sE[cur][1][0][tx + 2] - eyh) + chy * (sE[cur][0][1][tx + 1] - exh);
-            }
-            sH[0][0][tx + 1] = hrx;
-            sH[2][0][tx + 1] = hrz;
+        if (w_rlo) {  // (i, j0-1): Hx, Hz
+            const float ezh = sE[cur][2][0][tx + 1], eyh = sE[cur][1][0][tx + 1], exh = sE[cur][0][0][tx + 1];
+            const float vx = cu.hh0 - chy * (sE[cur][2][1][tx + 1] - ezh) + chy * (sE[nxt][1][0][tx + 1] - eyh);
+            const float vz = cu.hh1 - chx * (sE[cur][1][0][tx + 2] - eyh) + chy * (sE[cur][0][1][tx + 1] - exh);
+            sH[0][0][tx + 1] = (rx_ij && kh) ? vx : cu.hh0;
+            sH[2][0][tx + 1] = (rz_ij && kz) ? vz : cu.hh1;
         }
-        if (tx == 0) {
-            const int ih = i0 - 1;
-            if (col_halo) {
-                const float exh = sE[cur][0][ty + 1][0], eyh = sE[cur][1][ty + 1][0], ezh = sE[cur][2][ty + 1][0];
-                if (ih < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1)
-                    hcy = hcy - chz * (sE[nxt][0][ty + 1][0] - exh) + chx * (sE[cur][2][ty + 1][1] - ezh);
-                if (ih < nx - 1 && j < ny - 1 && k >= 1 && k < nz - 1)
-                    hcz = hcz - chx * (sE[cur][1][ty + 1][1] - eyh) + chy * (sE[cur][0][ty + 2][0] - exh);
-            }
-            sH[1][ty + 1][0] = hcy;
-            sH[2][ty + 1][0] = hcz;
+        if (w_col) {  // (i0-1, j0-1+ccr): Hy, Hz
+            const float exh = sE[cur][0][ccr][0], eyh = sE[cur][1][ccr][0], ezh = sE[cur][2][ccr][0];
+            const float vy = cu.hh0 - chz * (sE[nxt][0][ccr][0] - exh) + chx * (sE[cur][2][ccr][1] - ezh);
+            const float vz = cu.hh1 - chx * (sE[cur][1][ccr][1] - eyh) + chy * (sE[cur][0][ccr + 1][0] - exh);
+            *col_dst_y = (cy_ij && kh) ? vy : cu.hh0;
+            *col_dst_z = (cz_ij && kz) ? vz : cu.hh1;
         }
+        const uint32_t ob = (uint32_t)k * szb + c_own;
         if (own) {
-            stg(A.hxn, ob, hx); stg(A.hyn, ob, hy); stg(A.hzn, ob, hz);
+            stg_nt(hxn, ob, hx); stg_nt(hyn, ob, hy); stg_nt(hzn, ob, hz);
         }
         __syncthreads();  // H^{n+1/2} of plane k and its halo staged
 
+        const bool rdx = dx_ij && kz, rdy = dy_ij && kz, rdz = dz_ij && kh;
+        const bool reyz = e_i && (k >= 1);
+        const float dxu = cu.dxn + cdy * (hz - sH[2][ty][tx + 1]) - cdz * (hy - hym);
+        const float dyu = cu.dyn + cdz * (hx - hxm) - cdx * (hz - sH[2][ty + 1][tx]);
+        float dzu = cu.dzn + cdx * (hy - sH[1][ty + 1][tx]) - cdy * (hx - sH[0][ty][tx + 1]);
+        dzu = (cu.id & kSourceBit) ? dzu - jzv : dzu;
+        float dyp = dyu, dzp = dzu;
+        // E cells whose D is not updated read the target slot's value.
+        if (own && reyz && !rdy) dyp = ldg(dyT, ob);
+        if (own && reyz && !rdz) dzp = ldg(dzT, ob);
+        const int m = cu.id & kMaterialMask;
+        const float ca = lA[m], cc = lC[m], cd = lD[m];
+        const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
+        const float exn = e_update(cc, cd, ca, g1, g2, g3, exk, cu.exm, dxu, cu.dxn, cu.dxm);
+        const float eyn = e_update(cc, cd, ca, g1, g2, g3, eyk, cu.eym, dyp, cu.dyn, cu.dym);
+        const float ezn = e_update(cc, cd, ca, g1, g2, g3, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
         if (own) {
-            const bool rdx = (i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1);
-            const bool rdy = (i >= 1 && i < nx - 1 && j < ny - 1 && k >= 1 && k < nz - 1);
-            const bool rdz = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1);
-            const bool reyz = (i >= 1 && k >= 1);
-            float dxp = 0.f, dyp = 0.f, dzp = 0.f;
-            if (rdx) dxp = dxn + cdy * (hz - sH[2][ty][tx + 1]) - cdz * (hy - hym);
-            if (rdy) dyp = dyn + cdz * (hx - hxm) - cdx * (hz - sH[2][ty + 1][tx]);
-            if (rdz) {
-                dzp = dzn + cdx * (hy - sH[1][ty + 1][tx]) - cdy * (hx - sH[0][ty][tx + 1]);
-                if (id & kSourceBit) dzp = dzp - A.jz;
-            }
-            // Cells the D update skips keep the target slot's value.
-            if (reyz && !rdy) dyp = ldg(A.dyT, ob);
-            if (reyz && !rdz) dzp = ldg(A.dzT, ob);
-            if (rdx) stg(A.dxT, ob, dxp);
-            if (rdy) stg(A.dyT, ob, dyp);
-            if (rdz) stg(A.dzT, ob, dzp);
-            const int m = id & kMaterialMask;
-            const float ca = lA[m], cc = lC[m], cd = lD[m];
-            const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
-            if (rdx) stg(A.exT, ob, e_update(cc, cd, ca, g1, g2, g3, exk, exm, dxp, dxn, dxm));
-            if (reyz) {
-                stg(A.eyT, ob, e_update(cc, cd, ca, g1, g2, g3, eyk, eym, dyp, dyn, dym));
-                stg(A.ezT, ob, e_update(cc, cd, ca, g1, g2, g3, ezk, ezm, dzp, dzn, dzm));
-            }
+            if (rdx) { stg_nt(dxT, ob, dxu); stg_nt(exT, ob, exn); }
+            if (rdy) stg_nt(dyT, ob, dyu);
+            if (rdz) stg_nt(dzT, ob, dzu);
+            if (reyz) { stg_nt(eyT, ob, eyn); stg_nt(ezT, ob, ezn); }
         }
         hxm = hx;
         hym = hy;
         cur = nxt;
+        if (more) cu = nx_;
     }
 }
 
@@ -398,11 +428,11 @@ hipError_t launch_probes(const ProbeArgs &a, hipStream_t s)
 hipError_t launch_fused_step(const StepArgs &a, int ty, hipStream_t s)
 {
     const unsigned nblk = (unsigned)a.tiles_x * a.tiles_y * a.tiles_z;
+    const size_t lds = (size_t)a.nmat * (3 * sizeof(double) + 3 * sizeof(float));
     switch (ty) {
-    case 1: fused_step<1><<<nblk, dim3(TX, 1), 0, s>>>(a); break;
-    case 2: fused_step<2><<<nblk, dim3(TX, 2), 0, s>>>(a); break;
-    case 4: fused_step<4><<<nblk, dim3(TX, 4), 0, s>>>(a); break;
-    case 8: fused_step<8><<<nblk, dim3(TX, 8), 0, s>>>(a); break;
+    case 2: fused_step<2><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
+    case 4: fused_step<4><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
+    case 8: fused_step<8><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -50,9 +50,9 @@ typedef struct
     int debye_mode;     /* FDTD_DEBYE_FIXED (default) or FDTD_DEBYE_REF */
     int jz_index_mode;  /* FDTD_JZ_STEP (default) or FDTD_JZ_HOLD3 */
     int jz_limit;       /* jz entries used; 0 = the whole table, 512 = the reference's device table */
-    int tile_y;         /* rows per fused tile: 0 = auto, else 1, 2, 4 or 8 */
-    int kchunk;         /* z planes per fused block: 0 = auto */
-    int xcd_remap;      /* 1 (default): contiguous tile runs per XCD */
+    int tile_y;         /* rows per fused tile: 0 = auto (4), else 2, 4 or 8 */
+    int kchunk;         /* z planes per fused block: 0 = auto (8) */
+    int xcd_remap;      /* 1: contiguous tile runs per XCD (default 0) */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;

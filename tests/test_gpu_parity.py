@@ -120,7 +120,7 @@ def test_legacy_entry_points_bit_exact(debye):
 FUSED_CASES = [
     # shape, tile_y, kchunk, debye, jz mode
     ((37, 29, 23), 4, 0, O.DEBYE_FIXED, F.JZ_STEP),
-    ((37, 29, 23), 1, 5, O.DEBYE_REF, F.JZ_HOLD3),
+    ((37, 29, 23), 2, 5, O.DEBYE_REF, F.JZ_HOLD3),
     ((64, 64, 64), 8, 7, O.DEBYE_FIXED, F.JZ_STEP),
     ((130, 17, 9), 2, 3, O.DEBYE_FIXED, F.JZ_HOLD3),
     ((5, 6, 7), 4, 0, O.DEBYE_FIXED, F.JZ_STEP),

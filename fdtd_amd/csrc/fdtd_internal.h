@@ -47,7 +47,7 @@ struct StepArgs {
     float cdx, cdy, cdz;            // dt/dx, dt/dy, dt/dz
     int kchunk;                     // z planes per block
     int tiles_x, tiles_y, tiles_z;  // logical tile grid
-    int xcd_remap;                  // 1: map logical tiles to XCDs in contiguous runs
+    int xcd_remap;                  // 0: identity, 1: contiguous runs per XCD, 2: layer-interleaved
     unsigned guard;                 // addressable elements before and after every field array (and ids)
 };
 

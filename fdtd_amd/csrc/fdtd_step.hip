@@ -27,14 +27,27 @@ namespace {
 
 constexpr int TX = 64;
 
-__device__ inline int xcd_logical(int b, int nblk)
+__device__ inline int xcd_logical(int b, int nblk, int mode, int unit)
 {
-    // Workgroups are dispatched to the 8 XCDs round-robin by id.  Give each
-    // XCD a contiguous run of logical tiles so tiles that share halo planes
-    // share an L2.
-    const int q = nblk / 8, rem = nblk % 8;
-    const int x = b % 8, s = b / 8;
-    return x * q + min(x, rem) + s;
+    // Workgroups are dispatched to the 8 XCDs round-robin by id (b % 8).
+    if (mode == 1) {
+        // Each XCD takes one contiguous run of logical tiles.
+        const int q = nblk / 8, rem = nblk % 8;
+        const int x = b % 8, s = b / 8;
+        return x * q + min(x, rem) + s;
+    }
+    if (mode == 2) {
+        // Groups of 8 units (a unit = one z-chunk layer of tiles): XCD x
+        // takes unit x of each group, so the 8 XCDs work on 8 adjacent
+        // layers while the tiles of a layer (which share halo rows and
+        // columns) share one L2.  A partial last group keeps the identity.
+        const int G = 8 * unit;
+        const int full = (nblk / G) * G;
+        if (b >= full) return b;
+        const int g = b / G, r = b % G;
+        return g * G + (r % 8) * unit + r / 8;
+    }
+    return b;
 }
 
 __device__ inline float e_update(float c, float d, float a, double g1, double g2, double g3, float e0, float e1,
@@ -89,7 +102,7 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
-    const int b = A.xcd_remap ? xcd_logical((int)blockIdx.x, nblk) : (int)blockIdx.x;
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
     const int bx = b % A.tiles_x;
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);

@@ -52,7 +52,8 @@ typedef struct
     int jz_limit;       /* jz entries used; 0 = the whole table, 512 = the reference's device table */
     int tile_y;         /* rows per fused tile: 0 = auto (4), else 2, 4 or 8 */
     int kchunk;         /* z planes per fused block: 0 = auto (8) */
-    int xcd_remap;      /* 1: contiguous tile runs per XCD (default 0) */
+    int xcd_remap;      /* block -> tile order: 0 identity (default), 1 one contiguous run
+                           per XCD, 2 adjacent z-chunk layers on the 8 XCDs */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;

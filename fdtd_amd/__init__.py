@@ -26,6 +26,10 @@ DEBYE_REF = 0
 DEBYE_FIXED = 1
 JZ_HOLD3 = 0
 JZ_STEP = 1
+STORE_F32 = 0
+STORE_F16 = 1
+STORE_BF16 = 2
+STORE_F32_FAST = 3
 FIELDS = ["Dx", "Dy", "Dz", "Hx", "Hy", "Hz", "Ex", "Ey", "Ez"]
 
 # Every symbol the headers in include/ declare.
@@ -41,7 +45,7 @@ EXPORTS = {
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
-                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time"],
+                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales"],
 }
 
 
@@ -76,7 +80,8 @@ class FdtdField(C.Structure):
 
 class FdtdExt(C.Structure):
     _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
-                ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int)]
+                ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int), ("storage", C.c_int),
+                ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float)]
 
 
 class FdtdError(RuntimeError):
@@ -116,6 +121,7 @@ def lib():
         L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
         L.fdtd_ctx_set_timing.argtypes = [vp, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
         L.copySymbolsToDevice.argtypes = [C.POINTER(FdtdParams)]
         L.updateHField.argtypes = [vp] * 7
         L.updateDField.argtypes = [vp] * 10
@@ -165,12 +171,15 @@ def make_params(shape, dt, dx, dy=None, dz=None, mu0=None, eps0=None):
 class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
-    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0):
+    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0,
+                 storage=STORE_F32, scales=(0.0, 0.0, 0.0)):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))
         ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
         ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
+        ext.storage = storage
+        ext.scale_e, ext.scale_d, ext.scale_h = scales
         h = C.c_void_p()
         self._params = params
         check(L.fdtd_ctx_create(C.byref(params), C.byref(ext), C.byref(h)), "fdtd_ctx_create")
@@ -264,6 +273,11 @@ class Context:
         ms, n = C.c_double(), C.c_int()
         check(lib().fdtd_ctx_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def scales(self):
+        v = [C.c_double() for _ in range(3)]
+        check(lib().fdtd_ctx_scales(self._h, *[C.byref(x) for x in v]), "scales")
+        return tuple(x.value for x in v)
 
     def launch_shape(self):
         v = [C.c_int() for _ in range(4)]

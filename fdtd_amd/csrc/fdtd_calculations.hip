@@ -238,5 +238,5 @@ extern "C" int updateMurBoundary(float *exTarget, float *eyTarget, float *ezTarg
     m.nx = h_p.nx; m.ny = h_p.ny; m.nz = h_p.nz; m.pitch = h_p.nx; m.plane = (long long)h_p.nx * h_p.ny;
     m.dt = h_p.dt; m.hx = h_p.dx; m.hy = h_p.dy; m.hz = h_p.dz; m.mu0 = h_p.mu0; m.eps0 = h_p.eps0;
     mur_segments(h_p.nx, h_p.ny, h_p.nz, m.seg_start);
-    return set_err(launch_mur(m, stream));
+    return set_err(launch_mur(m, kStoreF32, stream));
 }

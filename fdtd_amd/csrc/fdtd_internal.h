@@ -15,6 +15,10 @@
 struct MatCoef {
     float a, c, d, pad;
     double g1, g2, g3;
+    // Scaled-unit form for 16-bit storage (fields stored as E/sE, (D/eps0)/sD,
+    // (eta0 H)/sH; all arithmetic fp32):
+    // E' = k1 E - k2 E_prev + k3 (D' - D) + k5 ((D' - D) - (D - D_prev)).
+    float k1, k2, k3, k5;
 };
 
 // Material id byte: low 7 bits index the coefficient table, the top bit marks
@@ -26,16 +30,21 @@ constexpr uint8_t kMaterialMask = 0x7f;
 // Host-side coefficient builder shared by the legacy and fused paths.
 MatCoef fdtd_mat_coef(float dt, float eps0, float sigma, float epsS, float epsI, float tau, int debye_mode);
 
+// Field storage of a context.
+enum Storage { kStoreF32 = 0, kStoreF16 = 1, kStoreBF16 = 2, kStoreF32Fast = 3 };
+
 // Arguments of the fused H -> D (+source) -> E kernel for one time step.
+// Field pointers point at element type float, __half or __hip_bfloat16
+// according to the launch's storage.
 struct StepArgs {
-    const float *hxo, *hyo, *hzo;   // H^{n-1/2}
-    float *hxn, *hyn, *hzn;         // H^{n+1/2}
-    const float *ex0, *ey0, *ez0;   // E^n
-    const float *ex1, *ey1, *ez1;   // E^{n-1}
-    float *exT, *eyT, *ezT;         // E^{n+1} (ring target slot)
-    const float *dx0, *dy0, *dz0;   // D^n
-    const float *dx1, *dy1, *dz1;   // D^{n-1}
-    float *dxT, *dyT, *dzT;         // D^{n+1} (ring target slot)
+    const void *hxo, *hyo, *hzo;    // H^{n-1/2}
+    void *hxn, *hyn, *hzn;          // H^{n+1/2}
+    const void *ex0, *ey0, *ez0;    // E^n
+    const void *ex1, *ey1, *ez1;    // E^{n-1}
+    void *exT, *eyT, *ezT;          // E^{n+1} (ring target slot)
+    const void *dx0, *dy0, *dz0;    // D^n
+    const void *dx1, *dy1, *dz1;    // D^{n-1}
+    void *dxT, *dyT, *dzT;          // D^{n+1} (ring target slot)
     const uint8_t *ids;
     const MatCoef *lut;
     int nmat;
@@ -58,8 +67,8 @@ inline long long fused_guard(int pitch, long long plane) { return plane + 10LL *
 
 // Arguments of the 6-face first-order Mur kernel (fdtd_calculations.cu:216-428).
 struct MurArgs {
-    float *exT, *eyT, *ezT;              // E^{n+1}
-    const float *exS, *eyS, *ezS;        // E^n
+    void *exT, *eyT, *ezT;               // E^{n+1}
+    const void *exS, *eyS, *ezS;         // E^n
     const float *rpx0, *rpy0, *rpz0, *rpxEnd, *rpyEnd, *rpzEnd;
     int nx, ny, nz, pitch;
     long long plane;
@@ -68,7 +77,8 @@ struct MurArgs {
 };
 
 struct ProbeArgs {
-    const float *field[9];               // Dx Dy Dz Hx Hy Hz Ex Ey Ez of the newest level
+    const void *field[9];                // Dx Dy Dz Hx Hy Hz Ex Ey Ez of the newest level
+    double factor[9];                    // stored value -> SI (1 for fp32 storage)
     const int *probes;                   // nprobe x (field, i, j, k)
     int nprobe;
     int pitch;
@@ -77,7 +87,7 @@ struct ProbeArgs {
     int row;
 };
 
-hipError_t launch_fused_step(const StepArgs &a, int ty, hipStream_t s);
-hipError_t launch_mur(const MurArgs &a, hipStream_t s);
-hipError_t launch_probes(const ProbeArgs &a, hipStream_t s);
+hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
+hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
+hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
 void mur_segments(int nx, int ny, int nz, int *seg_start);

@@ -179,9 +179,9 @@ int main(int argc, char **argv)
             if ((it + 1) % opt.every == 0) {
                 std::vector<const float *> dev(9);
                 for (int f = 0; f < 9; f++) {
-                    float *p = nullptr;
+                    void *p = nullptr;
                     CHECK_RC(fdtd_ctx_field_ptr(ctx, f, 0, &p));
-                    dev[f] = p;
+                    dev[f] = (const float *)p;
                 }
                 take(dev);
                 write_set(params, host.data(), it, opt.lines);

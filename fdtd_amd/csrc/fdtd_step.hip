@@ -17,9 +17,12 @@
 //     a uint8 id per cell (top bit = source cell), staged in LDS.
 // mur_faces: the 12 (component, face) first-order Mur updates
 // (fdtd_calculations.cu:216-428) over face cells only.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fdtd_internal.h"
 
@@ -60,23 +63,78 @@ __device__ inline float e_update(float c, float d, float a, double g1, double g2
     return (float)((double)c * s);
 }
 
-__device__ __forceinline__ float ldg(const float *p, uint32_t boff)
-{
-    return *(const float *)((const char *)p + boff);
-}
+// Element access at a 32-bit byte offset from a scalar base, for the three
+// storage types.  fp32 values pass through; fp16/bf16 convert to and from
+// fp32 with round-to-nearest-even.  The _nt forms are the streaming
+// (nontemporal) variants for data a launch touches once: the ring levels n
+// and n-1 of D, level n-1 of E, the ids and every store.
+template <typename T> struct Elem;
 
-__device__ __forceinline__ void stg(float *p, uint32_t boff, float v) { *(float *)((char *)p + boff) = v; }
+template <> struct Elem<float> {
+    static __device__ __forceinline__ float ld(const float *p, uint32_t b) { return *(const float *)((const char *)p + b); }
+    static __device__ __forceinline__ float ld_nt(const float *p, uint32_t b)
+    {
+        return __builtin_nontemporal_load((const float *)((const char *)p + b));
+    }
+    static __device__ __forceinline__ void st(float *p, uint32_t b, float v) { *(float *)((char *)p + b) = v; }
+    static __device__ __forceinline__ void st_nt(float *p, uint32_t b, float v)
+    {
+        __builtin_nontemporal_store(v, (float *)((char *)p + b));
+    }
+};
 
-// Streaming (nontemporal) forms for data each launch touches once: the
-// ring levels n and n-1 of D, level n-1 of E, the ids and every store.
-__device__ __forceinline__ float ldg_nt(const float *p, uint32_t boff)
-{
-    return __builtin_nontemporal_load((const float *)((const char *)p + boff));
-}
+template <> struct Elem<__half> {
+    static __device__ __forceinline__ float cvt(unsigned short u) { return __half2float(__ushort_as_half(u)); }
+    static __device__ __forceinline__ unsigned short enc(float v) { return __half_as_ushort(__float2half_rn(v)); }
+    static __device__ __forceinline__ float ld(const __half *p, uint32_t b)
+    {
+        return cvt(*(const unsigned short *)((const char *)p + b));
+    }
+    static __device__ __forceinline__ float ld_nt(const __half *p, uint32_t b)
+    {
+        return cvt(__builtin_nontemporal_load((const unsigned short *)((const char *)p + b)));
+    }
+    static __device__ __forceinline__ void st(__half *p, uint32_t b, float v) { *(unsigned short *)((char *)p + b) = enc(v); }
+    static __device__ __forceinline__ void st_nt(__half *p, uint32_t b, float v)
+    {
+        __builtin_nontemporal_store(enc(v), (unsigned short *)((char *)p + b));
+    }
+};
 
-__device__ __forceinline__ void stg_nt(float *p, uint32_t boff, float v)
+template <> struct Elem<__hip_bfloat16> {
+    static __device__ __forceinline__ float cvt(unsigned short u) { return __uint_as_float((uint32_t)u << 16); }
+    static __device__ __forceinline__ unsigned short enc(float v)
+    {
+        // round to nearest even; NaN stays NaN
+        const uint32_t x = __float_as_uint(v);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((x >> 16) | 0x40u);
+        return (unsigned short)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+    }
+    static __device__ __forceinline__ float ld(const __hip_bfloat16 *p, uint32_t b)
+    {
+        return cvt(*(const unsigned short *)((const char *)p + b));
+    }
+    static __device__ __forceinline__ float ld_nt(const __hip_bfloat16 *p, uint32_t b)
+    {
+        return cvt(__builtin_nontemporal_load((const unsigned short *)((const char *)p + b)));
+    }
+    static __device__ __forceinline__ void st(__hip_bfloat16 *p, uint32_t b, float v)
+    {
+        *(unsigned short *)((char *)p + b) = enc(v);
+    }
+    static __device__ __forceinline__ void st_nt(__hip_bfloat16 *p, uint32_t b, float v)
+    {
+        __builtin_nontemporal_store(enc(v), (unsigned short *)((char *)p + b));
+    }
+};
+
+// E^{n+1} in scaled units (16-bit storage): fp32 throughout.
+__device__ inline float e_update_scaled(float k1, float k2, float k3, float k5, float e0, float e1, float dn1,
+                                        float dn, float dm1)
 {
-    __builtin_nontemporal_store(v, (float *)((char *)p + boff));
+    const float d1 = dn1 - dn;
+    const float d2 = d1 - (dn - dm1);
+    return k1 * e0 - k2 * e1 + k3 * d1 + k5 * d2;
 }
 
 // All per-lane addresses are a 64-bit array base (uniform, in SGPRs) plus a
@@ -91,14 +149,20 @@ __device__ __forceinline__ void stg_nt(float *p, uint32_t boff, float v)
 // -y halo row, wave TY-1 loads the +y E row, and wave 1 gathers the -x/+x
 // E columns and updates the -x halo H column (lane l: side l/32, row l%32).
 // Each plane's global loads are issued one plane ahead of their use.
-template <int TY>
+template <int TY, typename T, bool EXACT>
 __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
 {
     static_assert(TY >= 2 && TY <= 8, "tile rows");
+    static_assert(!EXACT || std::is_same<T, float>::value, "the exact form stores fp32");
+    // EXACT reproduces the reference's arithmetic (double Debye sums); the
+    // other form runs scaled units in fp32 (16-bit storage, fast fp32).
+    constexpr bool exact = EXACT;
+    constexpr uint32_t ES = sizeof(T);
+    using X = Elem<T>;
     __shared__ float sE[2][3][TY + 2][TX + 2];
     __shared__ float sH[3][TY + 1][TX + 1];
     __shared__ float sJunk[2][TX];
-    extern __shared__ double sLut[];  // g1[nmat] g2[nmat] g3[nmat] then float a[nmat] c[nmat] d[nmat]
+    extern __shared__ double sLut[];  // exact: g1 g2 g3 [nmat] doubles, a c d floats; scaled: k1 k2 k3 k5 floats
 
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
@@ -117,35 +181,41 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     const int nmat = A.nmat;
     double *lG1 = sLut, *lG2 = sLut + nmat, *lG3 = sLut + 2 * nmat;
     float *lA = (float *)(sLut + 3 * nmat), *lC = lA + nmat, *lD = lA + 2 * nmat;
+    float *lK1 = (float *)sLut, *lK2 = lK1 + nmat, *lK3 = lK1 + 2 * nmat, *lK5 = lK1 + 3 * nmat;
     for (int m = tid; m < nmat; m += TX * TY) {
         const MatCoef mc = A.lut[m];
-        lA[m] = mc.a; lC[m] = mc.c; lD[m] = mc.d;
-        lG1[m] = mc.g1; lG2[m] = mc.g2; lG3[m] = mc.g3;
+        if constexpr (exact) {
+            lA[m] = mc.a; lC[m] = mc.c; lD[m] = mc.d;
+            lG1[m] = mc.g1; lG2[m] = mc.g2; lG3[m] = mc.g3;
+        } else {
+            lK1[m] = mc.k1; lK2[m] = mc.k2; lK3[m] = mc.k3; lK5[m] = mc.k5;
+        }
     }
 
     // Origin-shifted bases (guard elements before each array's origin).
-    const uint32_t gb = A.guard * 4u;
-    auto sh = [&](const float *p) { return (const float *)((const char *)p - gb); };
-    const float *hxo = sh(A.hxo), *hyo = sh(A.hyo), *hzo = sh(A.hzo);
-    const float *ex0 = sh(A.ex0), *ey0 = sh(A.ey0), *ez0 = sh(A.ez0);
-    const float *ex1 = sh(A.ex1), *ey1 = sh(A.ey1), *ez1 = sh(A.ez1);
-    const float *dx0 = sh(A.dx0), *dy0 = sh(A.dy0), *dz0 = sh(A.dz0);
-    const float *dx1 = sh(A.dx1), *dy1 = sh(A.dy1), *dz1 = sh(A.dz1);
-    float *hxn = (float *)sh(A.hxn), *hyn = (float *)sh(A.hyn), *hzn = (float *)sh(A.hzn);
-    float *exT = (float *)sh(A.exT), *eyT = (float *)sh(A.eyT), *ezT = (float *)sh(A.ezT);
-    float *dxT = (float *)sh(A.dxT), *dyT = (float *)sh(A.dyT), *dzT = (float *)sh(A.dzT);
+    const uint32_t gb = A.guard * ES;
+    auto sh = [&](const void *p) { return (const T *)((const char *)p - gb); };
+    auto shw = [&](void *p) { return (T *)((char *)p - gb); };
+    const T *hxo = sh(A.hxo), *hyo = sh(A.hyo), *hzo = sh(A.hzo);
+    const T *ex0 = sh(A.ex0), *ey0 = sh(A.ey0), *ez0 = sh(A.ez0);
+    const T *ex1 = sh(A.ex1), *ey1 = sh(A.ey1), *ez1 = sh(A.ez1);
+    const T *dx0 = sh(A.dx0), *dy0 = sh(A.dy0), *dz0 = sh(A.dz0);
+    const T *dx1 = sh(A.dx1), *dy1 = sh(A.dy1), *dz1 = sh(A.dz1);
+    T *hxn = shw(A.hxn), *hyn = shw(A.hyn), *hzn = shw(A.hzn);
+    T *exT = shw(A.exT), *eyT = shw(A.eyT), *ezT = shw(A.ezT);
+    T *dxT = shw(A.dxT), *dyT = shw(A.dyT), *dzT = shw(A.dzT);
     const uint8_t *ids = A.ids - A.guard;
 
-    const uint32_t syb = (uint32_t)A.pitch * 4u, szb = (uint32_t)A.plane * 4u;
+    const uint32_t syb = (uint32_t)A.pitch * ES, szb = (uint32_t)A.plane * ES;
     // plane-relative byte offsets (wrap-around arithmetic handles j0-1 = -1 etc.)
-    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)i * 4u;
-    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)i * 4u;
-    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)i * 4u;
+    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)i * ES;
+    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)i * ES;
+    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)i * ES;
     const int side = tx >> 5;
     const int cr = min(tx & 31, TY + 1);             // LDS row of the gathered column cell
     const int cl = side ? TX + 1 : 0;                 // LDS column of the gathered cell
     const int ci = side ? i0 + TX : i0 - 1;
-    const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * 4u;
+    const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * ES;
     const bool w_rlo = (ty == 0), w_rhi = (ty == TY - 1), w_col = (ty == 1);
 
     const float chx = A.chx, chy = A.chy, chz = A.chz;
@@ -156,10 +226,10 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     float pe[3], pr[3], pq[2], pc[3];
     auto fetch_e = [&](int k) {
         const uint32_t kb = (uint32_t)k * szb;
-        pe[0] = ldg(ex0, kb + c_own); pe[1] = ldg(ey0, kb + c_own); pe[2] = ldg(ez0, kb + c_own);
-        if (w_rlo) { pr[0] = ldg(ex0, kb + c_rlo); pr[1] = ldg(ey0, kb + c_rlo); pr[2] = ldg(ez0, kb + c_rlo); }
-        if (w_rhi) { pq[0] = ldg(ex0, kb + c_rhi); pq[1] = ldg(ez0, kb + c_rhi); }
-        if (w_col) { pc[0] = ldg(ex0, kb + c_col); pc[1] = ldg(ey0, kb + c_col); pc[2] = ldg(ez0, kb + c_col); }
+        pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
+        if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
+        if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
+        if (w_col) { pc[0] = X::ld(ex0, kb + c_col); pc[1] = X::ld(ey0, kb + c_col); pc[2] = X::ld(ez0, kb + c_col); }
     };
     auto stage_e = [&](int buf) {
         sE[buf][0][ty + 1][tx + 1] = pe[0]; sE[buf][1][ty + 1][tx + 1] = pe[1]; sE[buf][2][ty + 1][tx + 1] = pe[2];
@@ -175,15 +245,15 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     };
     auto fetch_own = [&](int k, Own &o) {
         const uint32_t ob = (uint32_t)k * szb + c_own;
-        o.hxo = ldg(hxo, ob); o.hyo = ldg(hyo, ob); o.hzo = ldg(hzo, ob);
-        o.dxn = ldg_nt(dx0, ob); o.dyn = ldg_nt(dy0, ob); o.dzn = ldg_nt(dz0, ob);
-        o.dxm = ldg_nt(dx1, ob); o.dym = ldg_nt(dy1, ob); o.dzm = ldg_nt(dz1, ob);
-        o.exm = ldg_nt(ex1, ob); o.eym = ldg_nt(ey1, ob); o.ezm = ldg_nt(ez1, ob);
-        o.id = __builtin_nontemporal_load(ids + (ob >> 2));
+        o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
+        o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
+        o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
+        o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+        o.id = __builtin_nontemporal_load(ids + ob / ES);
         // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TX, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;
-        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = ldg(hxo, oh); o.hh1 = ldg(hzo, oh); }
-        if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = ldg(hyo, oh); o.hh1 = ldg(hzo, oh); }
+        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
+        if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
     };
 
     float hxm = 0.0f, hym = 0.0f;  // own-column Hx, Hy of plane k-1 (time level n+1/2)
@@ -196,7 +266,7 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
         __syncthreads();
         const int k = k0 - 1;
         const uint32_t ob = (uint32_t)k * szb + c_own;
-        const float hx0 = ldg(hxo, ob), hy0 = ldg(hyo, ob);
+        const float hx0 = X::ld(hxo, ob), hy0 = X::ld(hyo, ob);
         const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
         const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
         const float ux = hx0 - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
@@ -273,7 +343,7 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
         }
         const uint32_t ob = (uint32_t)k * szb + c_own;
         if (own) {
-            stg_nt(hxn, ob, hx); stg_nt(hyn, ob, hy); stg_nt(hzn, ob, hz);
+            X::st_nt(hxn, ob, hx); X::st_nt(hyn, ob, hy); X::st_nt(hzn, ob, hz);
         }
         __syncthreads();  // H^{n+1/2} of plane k and its halo staged
 
@@ -285,19 +355,27 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
         dzu = (cu.id & kSourceBit) ? dzu - jzv : dzu;
         float dyp = dyu, dzp = dzu;
         // E cells whose D is not updated read the target slot's value.
-        if (own && reyz && !rdy) dyp = ldg(dyT, ob);
-        if (own && reyz && !rdz) dzp = ldg(dzT, ob);
+        if (own && reyz && !rdy) dyp = X::ld(dyT, ob);
+        if (own && reyz && !rdz) dzp = X::ld(dzT, ob);
         const int m = cu.id & kMaterialMask;
-        const float ca = lA[m], cc = lC[m], cd = lD[m];
-        const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
-        const float exn = e_update(cc, cd, ca, g1, g2, g3, exk, cu.exm, dxu, cu.dxn, cu.dxm);
-        const float eyn = e_update(cc, cd, ca, g1, g2, g3, eyk, cu.eym, dyp, cu.dyn, cu.dym);
-        const float ezn = e_update(cc, cd, ca, g1, g2, g3, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
+        float exn, eyn, ezn;
+        if constexpr (exact) {
+            const float ca = lA[m], cc = lC[m], cd = lD[m];
+            const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
+            exn = e_update(cc, cd, ca, g1, g2, g3, exk, cu.exm, dxu, cu.dxn, cu.dxm);
+            eyn = e_update(cc, cd, ca, g1, g2, g3, eyk, cu.eym, dyp, cu.dyn, cu.dym);
+            ezn = e_update(cc, cd, ca, g1, g2, g3, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
+        } else {
+            const float k1 = lK1[m], k2 = lK2[m], k3 = lK3[m], k5 = lK5[m];
+            exn = e_update_scaled(k1, k2, k3, k5, exk, cu.exm, dxu, cu.dxn, cu.dxm);
+            eyn = e_update_scaled(k1, k2, k3, k5, eyk, cu.eym, dyp, cu.dyn, cu.dym);
+            ezn = e_update_scaled(k1, k2, k3, k5, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
+        }
         if (own) {
-            if (rdx) { stg_nt(dxT, ob, dxu); stg_nt(exT, ob, exn); }
-            if (rdy) stg_nt(dyT, ob, dyu);
-            if (rdz) stg_nt(dzT, ob, dzu);
-            if (reyz) { stg_nt(eyT, ob, eyn); stg_nt(ezT, ob, ezn); }
+            if (rdx) { X::st_nt(dxT, ob, dxu); X::st_nt(exT, ob, exn); }
+            if (rdy) X::st_nt(dyT, ob, dyu);
+            if (rdz) X::st_nt(dzT, ob, dzu);
+            if (reyz) { X::st_nt(eyT, ob, eyn); X::st_nt(ezT, ob, ezn); }
         }
         hxm = hx;
         hym = hy;
@@ -316,8 +394,12 @@ __device__ inline float mur1(float dt, float h, float mu0, float eps0, float rpf
 
 constexpr int kMurBlock = 256;
 
+template <typename T>
 __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
 {
+    using X = Elem<T>;
+    T *exT = (T *)M.exT, *eyT = (T *)M.eyT, *ezT = (T *)M.ezT;
+    const T *exS = (const T *)M.exS, *eyS = (const T *)M.eyS, *ezS = (const T *)M.ezS;
     const int t = blockIdx.x * kMurBlock + threadIdx.x;
     if (t >= M.seg_start[12]) return;
     int s = 0;
@@ -328,6 +410,8 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
     const long long sy = M.pitch, sz = M.plane;
     const float dt = M.dt, m0 = M.mu0, e0 = M.eps0;
     auto at = [&](int i, int j, int k) { return (long long)k * sz + (long long)j * sy + i; };
+    auto G = [&](const T *p, long long o) { return X::ld(p, (uint32_t)(o * (long long)sizeof(T))); };
+    auto P = [&](T *p, long long o, float v) { X::st(p, (uint32_t)(o * (long long)sizeof(T)), v); };
     auto rpy = [&](const float *p, int i, int jj, int k) { return p[(long long)k * 2 * nx + (long long)jj * nx + i]; };
     auto rpz = [&](const float *p, int i, int j, int kk) { return p[(long long)kk * ny * nx + (long long)j * nx + i]; };
     auto rpx = [&](const float *p, int ii, int j, int k) { return p[(long long)k * ny * 2 + (long long)j * 2 + ii]; };
@@ -336,10 +420,10 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int i = q % (nx - 1), k = 1 + q / (nx - 1);
         if (s == 0) {
             const long long o = at(i, 0, k);
-            M.exT[o] = mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), M.exT[o + sy], M.exS[o + sy], M.exS[o]);
+            P(exT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), G(exT, o + sy), G(exS, o + sy), G(exS, o)));
         } else {
             const long long o = at(i, ny - 1, k);
-            M.exT[o] = mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), M.exT[o - sy], M.exS[o - sy], M.exS[o]);
+            P(exT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), G(exT, o - sy), G(exS, o - sy), G(exS, o)));
         }
         break;
     }
@@ -347,10 +431,10 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int i = q % (nx - 1), j = 1 + q / (nx - 1);
         if (s == 2) {
             const long long o = at(i, j, 0);
-            M.exT[o] = mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), M.exT[o + sz], M.exS[o + sz], M.exS[o]);
+            P(exT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), G(exT, o + sz), G(exS, o + sz), G(exS, o)));
         } else {
             const long long o = at(i, j, nz - 1);
-            M.exT[o] = mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), M.exT[o - sz], M.exS[o - sz], M.exS[o]);
+            P(exT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), G(exT, o - sz), G(exS, o - sz), G(exS, o)));
         }
         break;
     }
@@ -358,10 +442,10 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int j = q % (ny - 1), k = 1 + q / (ny - 1);
         if (s == 4) {
             const long long o = at(0, j, k);
-            M.eyT[o] = mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), M.eyT[o + 1], M.eyS[o + 1], M.eyS[o]);
+            P(eyT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), G(eyT, o + 1), G(eyS, o + 1), G(eyS, o)));
         } else {  // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
             const long long o = at(nx - 1, j, k);
-            M.eyT[o] = mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), M.eyS[o - 1], M.eyS[o - 1], M.eyS[o]);
+            P(eyT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), G(eyS, o - 1), G(eyS, o - 1), G(eyS, o)));
         }
         break;
     }
@@ -369,10 +453,10 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int i = 1 + q % (nx - 2), j = q / (nx - 2);
         if (s == 6) {
             const long long o = at(i, j, 0);
-            M.eyT[o] = mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), M.eyT[o + sz], M.eyS[o + sz], M.eyS[o]);
+            P(eyT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), G(eyT, o + sz), G(eyS, o + sz), G(eyS, o)));
         } else {
             const long long o = at(i, j, nz - 1);
-            M.eyT[o] = mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), M.eyT[o - sz], M.eyS[o - sz], M.eyS[o]);
+            P(eyT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), G(eyT, o - sz), G(eyS, o - sz), G(eyS, o)));
         }
         break;
     }
@@ -380,10 +464,10 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int j = 1 + q % (ny - 2), k = q / (ny - 2);
         if (s == 8) {
             const long long o = at(0, j, k);
-            M.ezT[o] = mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), M.ezT[o + 1], M.ezS[o + 1], M.ezS[o]);
+            P(ezT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), G(ezT, o + 1), G(ezS, o + 1), G(ezS, o)));
         } else {
             const long long o = at(nx - 1, j, k);
-            M.ezT[o] = mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), M.ezT[o - 1], M.ezS[o - 1], M.ezS[o]);
+            P(ezT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), G(ezT, o - 1), G(ezS, o - 1), G(ezS, o)));
         }
         break;
     }
@@ -391,23 +475,25 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         const int i = 1 + q % (nx - 2), k = q / (nx - 2);
         if (s == 10) {
             const long long o = at(i, 0, k);
-            M.ezT[o] = mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), M.ezT[o + sy], M.ezS[o + sy], M.ezS[o]);
+            P(ezT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), G(ezT, o + sy), G(ezS, o + sy), G(ezS, o)));
         } else {
             const long long o = at(i, ny - 1, k);
-            M.ezT[o] = mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), M.ezT[o - sy], M.ezS[o - sy], M.ezS[o]);
+            P(ezT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), G(ezT, o - sy), G(ezS, o - sy), G(ezS, o)));
         }
         break;
     }
     }
 }
 
+template <typename T>
 __global__ void probe_cells(const ProbeArgs P)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.nprobe) return;
     const int *q = P.probes + 4 * p;
     const long long o = (long long)q[3] * P.plane + (long long)q[2] * P.pitch + q[1];
-    P.out[(long long)P.row * P.nprobe + p] = (double)P.field[q[0]][o];
+    const float v = Elem<T>::ld((const T *)P.field[q[0]], (uint32_t)(o * (long long)sizeof(T)));
+    P.out[(long long)P.row * P.nprobe + p] = (double)v * P.factor[q[0]];
 }
 
 }  // namespace
@@ -423,30 +509,60 @@ void mur_segments(int nx, int ny, int nz, int *seg)
     for (int q = 0; q < 12; q++) seg[q + 1] = seg[q] + std::max(cnt[q], 0);
 }
 
-hipError_t launch_mur(const MurArgs &a, hipStream_t s)
+hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s)
 {
     const int total = a.seg_start[12];
     if (total <= 0) return hipSuccess;
-    mur_faces<<<(total + kMurBlock - 1) / kMurBlock, kMurBlock, 0, s>>>(a);
+    const int grid = (total + kMurBlock - 1) / kMurBlock;
+    switch (storage) {
+    case kStoreF32:
+    case kStoreF32Fast: mur_faces<float><<<grid, kMurBlock, 0, s>>>(a); break;
+    case kStoreF16: mur_faces<__half><<<grid, kMurBlock, 0, s>>>(a); break;
+    case kStoreBF16: mur_faces<__hip_bfloat16><<<grid, kMurBlock, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_probes(const ProbeArgs &a, hipStream_t s)
+hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s)
 {
     if (a.nprobe <= 0) return hipSuccess;
-    probe_cells<<<(a.nprobe + 63) / 64, 64, 0, s>>>(a);
+    const int grid = (a.nprobe + 63) / 64;
+    switch (storage) {
+    case kStoreF32:
+    case kStoreF32Fast: probe_cells<float><<<grid, 64, 0, s>>>(a); break;
+    case kStoreF16: probe_cells<__half><<<grid, 64, 0, s>>>(a); break;
+    case kStoreBF16: probe_cells<__hip_bfloat16><<<grid, 64, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_fused_step(const StepArgs &a, int ty, hipStream_t s)
+namespace {
+
+template <typename T, bool EXACT>
+hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s)
 {
     const unsigned nblk = (unsigned)a.tiles_x * a.tiles_y * a.tiles_z;
     const size_t lds = (size_t)a.nmat * (3 * sizeof(double) + 3 * sizeof(float));
     switch (ty) {
-    case 2: fused_step<2><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
-    case 4: fused_step<4><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
-    case 8: fused_step<8><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
+    case 2: fused_step<2, T, EXACT><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
+    case 4: fused_step<4, T, EXACT><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
+    case 8: fused_step<8, T, EXACT><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s)
+{
+    switch (storage) {
+    case kStoreF32: return launch_fused_t<float, true>(a, ty, s);
+    case kStoreF32Fast: return launch_fused_t<float, false>(a, ty, s);
+    case kStoreF16: return launch_fused_t<__half, false>(a, ty, s);
+    case kStoreBF16: return launch_fused_t<__hip_bfloat16, false>(a, ty, s);
+    default: return hipErrorInvalidValue;
+    }
 }

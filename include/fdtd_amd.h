@@ -28,6 +28,11 @@ extern "C" {
 #define FDTD_JZ_HOLD3 0     /* step n uses jz[3*floor(n/3)] (fdtd.cu:172,310,448) */
 #define FDTD_JZ_STEP  1     /* step n uses jz[n] */
 
+#define FDTD_STORE_F32  0   /* fp32 fields: the reference's arithmetic, bit for bit */
+#define FDTD_STORE_F16  1   /* fp16 fields, fp32 arithmetic, per-field scales */
+#define FDTD_STORE_BF16 2   /* bf16 fields, fp32 arithmetic, per-field scales */
+#define FDTD_STORE_F32_FAST 3 /* fp32 fields, the scaled-unit fp32 arithmetic of the 16-bit forms */
+
 #define FDTD_EINVAL   (-1)  /* bad argument */
 #define FDTD_ERANGE   (-2)  /* index or size out of range */
 #define FDTD_ESTATE   (-3)  /* call out of order */
@@ -54,6 +59,10 @@ typedef struct
     int kchunk;         /* z planes per fused block: 0 = auto (8) */
     int xcd_remap;      /* block -> tile order: 0 identity (default), 1 one contiguous run
                            per XCD, 2 adjacent z-chunk layers on the 8 XCDs */
+    int storage;        /* FDTD_STORE_F32 (default), _F16, _BF16 or _F32_FAST */
+    float scale_e;      /* 16-bit storage: E stored as E / scale_e (0 = auto) */
+    float scale_d;      /* 16-bit storage: D stored as D / (eps0 scale_d) (0 = auto) */
+    float scale_h;      /* 16-bit storage: H stored as eta0 H / scale_h (0 = auto) */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;
@@ -93,8 +102,13 @@ int fdtd_ctx_set_slices(fdtd_ctx *ctx, int k, int interval, int capacity);
  * ring slot.  H has one level. */
 int fdtd_ctx_set_field(fdtd_ctx *ctx, int field, int level, const float *host);
 int fdtd_ctx_get_field(fdtd_ctx *ctx, int field, int level, float *host);
-/* Device address of a field (for zero-copy callers). */
-int fdtd_ctx_field_ptr(fdtd_ctx *ctx, int field, int level, float **dev);
+/* Device address of a field (for zero-copy callers); elements are float,
+ * or 16-bit scaled values under 16-bit storage (see fdtd_ctx_scales). */
+int fdtd_ctx_field_ptr(fdtd_ctx *ctx, int field, int level, void **dev);
+/* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales are 1/32 of
+ * the largest per-step D/eps0 kick of the source waveform, fixed at the
+ * first step. */
+int fdtd_ctx_scales(fdtd_ctx *ctx, double *scale_e, double *scale_d, double *scale_h);
 
 /* One full time step (H, D, sources, Debye E, Mur, probes) on `stream`. */
 int fdtd_step_fused(fdtd_ctx *ctx, hipStream_t stream);

@@ -7,7 +7,8 @@ Python binding of libfdtd_amd.so, whose C ABI is declared in include/:
     updateSources, updateMurBoundary), fdtd_calculations.h:18-42;
   * fdtd.h -- FdtdParams / FdtdField and the host services
     (initParamsWithPath, setupSources, setupMurBoundary, loadMaterials, ...);
-  * fdtd_amd.h -- the context API with the fused gfx950 step.
+  * fdtd_amd.h -- the context API with the fused gfx950 step, batches of
+    grids stepped together, and the serving loop.
 
 The library must be built (``python -c "import __graft_entry__ as g; g.build()"``
 or ``make -C fdtd_amd/csrc``); there is no CPU fallback: every compute call
@@ -45,8 +46,15 @@ EXPORTS = {
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
-                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales"],
+                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales",
+                   "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
+                   "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
+                   "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
+                   "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
+                   "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
+ENOMEM = -5
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class FdtdParams(C.Structure):
@@ -82,6 +90,39 @@ class FdtdExt(C.Structure):
     _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
                 ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int), ("storage", C.c_int),
                 ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float)]
+
+
+class FdtdInstance(C.Structure):
+    """One grid of a batch or a served request (fdtd_amd.h)."""
+    _fields_ = [("ids", C.c_void_p), ("nmat", C.c_int), ("table", C.c_void_p), ("rp", C.c_void_p * 6),
+                ("nsrc", C.c_int), ("src_xyz", C.c_void_p), ("jz_len", C.c_int), ("jz", C.c_void_p),
+                ("steps", C.c_int), ("nprobe", C.c_int), ("probes", C.c_void_p)]
+
+
+class FdtdBatchInfo(C.Structure):
+    _fields_ = [("instances", C.c_int), ("running", C.c_int), ("step", C.c_longlong),
+                ("arena_bytes", C.c_size_t), ("used_bytes", C.c_size_t), ("largest_free", C.c_size_t),
+                ("tile_y", C.c_int), ("kchunk", C.c_int), ("graph_steps", C.c_int),
+                ("grid", C.c_int * 4), ("tiles", C.c_int * 4), ("cells", C.c_longlong * 4)]
+
+
+class FdtdServeConfig(C.Structure):
+    _fields_ = [("hbm_bytes", C.c_size_t), ("max_slots", C.c_int), ("segment_steps", C.c_int),
+                ("tile_y", C.c_int), ("kchunk", C.c_int)]
+
+
+class FdtdRequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("error", C.c_int), ("rows", C.c_int), ("nprobe", C.c_int),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("cells", C.c_longlong), ("steps", C.c_int)]
+
+
+class FdtdServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int), ("completed", C.c_int), ("failed", C.c_int), ("queued", C.c_int),
+                ("running", C.c_int), ("t_first_submit", C.c_double), ("t_last_done", C.c_double),
+                ("solves_per_s", C.c_double), ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
+                ("mean_latency_s", C.c_double), ("cell_updates", C.c_double), ("segments", C.c_longlong),
+                ("peak_bytes", C.c_size_t), ("peak_running", C.c_int)]
 
 
 class FdtdError(RuntimeError):
@@ -122,6 +163,27 @@ def lib():
         L.fdtd_ctx_set_timing.argtypes = [vp, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
+        PP, PE, PI = C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(FdtdInstance)
+        L.fdtd_batch_create.argtypes = [C.c_size_t, C.POINTER(vp)]
+        L.fdtd_batch_destroy.argtypes = [vp]
+        L.fdtd_batch_instance_bytes.argtypes = [PP, PE, PI, C.POINTER(C.c_size_t)]
+        L.fdtd_batch_add.argtypes = [vp, PP, PE, PI, C.POINTER(C.c_int)]
+        L.fdtd_batch_remove.argtypes = [vp, i]
+        L.fdtd_batch_run.argtypes = [vp, i, vp]
+        L.fdtd_batch_status.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.fdtd_batch_probe_read.argtypes = [vp, i, vp, i]
+        L.fdtd_batch_get_field.argtypes = [vp, i, i, i, vp]
+        L.fdtd_batch_set_shape.argtypes = [vp, i, i, i]
+        L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
+        L.fdtd_serve_create.argtypes = [C.POINTER(FdtdServeConfig), C.POINTER(vp)]
+        L.fdtd_serve_destroy.argtypes = [vp]
+        L.fdtd_serve_submit.argtypes = [vp, PP, PE, PI, C.POINTER(C.c_longlong)]
+        L.fdtd_serve_wait.argtypes = [vp, C.c_longlong, C.c_double]
+        L.fdtd_serve_result.argtypes = [vp, C.c_longlong, vp, i, C.POINTER(FdtdRequestInfo)]
+        L.fdtd_serve_release.argtypes = [vp, C.c_longlong]
+        L.fdtd_serve_drain.argtypes = [vp, C.c_double]
+        L.fdtd_serve_stats.argtypes = [vp, C.POINTER(FdtdServeStats)]
         L.copySymbolsToDevice.argtypes = [C.POINTER(FdtdParams)]
         L.updateHField.argtypes = [vp] * 7
         L.updateDField.argtypes = [vp] * 10
@@ -283,3 +345,176 @@ class Context:
         v = [C.c_int() for _ in range(4)]
         check(lib().fdtd_ctx_launch_shape(self._h, *[C.byref(x) for x in v]), "launch_shape")
         return dict(tile_x=v[0].value, tile_y=v[1].value, kchunk=v[2].value, blocks=v[3].value)
+
+
+def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, scales=(0.0, 0.0, 0.0),
+             tile_y=0, kchunk=0, xcd_remap=0):
+    ext = FdtdExt()
+    lib().fdtd_ext_defaults(C.byref(ext))
+    ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
+    ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
+    ext.storage = storage
+    ext.scale_e, ext.scale_d, ext.scale_h = scales
+    return ext
+
+
+class Instance:
+    """Inputs of one batched or served grid; keeps the arrays alive while the
+    FdtdInstance that points at them is in use."""
+
+    def __init__(self, params, ext, ids, table, src_xyz, jz, steps, probes=(), rp=None):
+        self.params, self.ext = params, ext
+        self.shape = (params.nx, params.ny, params.nz)
+        self.ids = None if ids is None else np.ascontiguousarray(ids, np.uint8).ravel()
+        self.table = np.ascontiguousarray(table, np.float32).reshape(-1, 4)
+        self.src = np.ascontiguousarray(np.asarray(src_xyz, np.int32).reshape(-1, 3))
+        self.jz = np.ascontiguousarray(jz, np.float32)
+        self.probes = np.ascontiguousarray(np.asarray(probes, np.int32).reshape(-1, 4))
+        self.rp = None if rp is None else [np.ascontiguousarray(x, np.float32).ravel() for x in rp]
+        self.steps = int(steps)
+        s = FdtdInstance()
+        s.ids = _ptr(self.ids)
+        s.nmat = self.table.shape[0]
+        s.table = _ptr(self.table)
+        for r in range(6):
+            s.rp[r] = _ptr(self.rp[r]) if self.rp is not None else None
+        s.nsrc = self.src.shape[0]
+        s.src_xyz = _ptr(self.src)
+        s.jz_len = self.jz.size
+        s.jz = _ptr(self.jz)
+        s.steps = self.steps
+        s.nprobe = self.probes.shape[0]
+        s.probes = _ptr(self.probes)
+        self.c = s
+
+    @property
+    def nprobe(self):
+        return self.probes.shape[0]
+
+    def nbytes(self):
+        b = C.c_size_t()
+        check(lib().fdtd_batch_instance_bytes(C.byref(self.params), C.byref(self.ext), C.byref(self.c),
+                                              C.byref(b)), "instance_bytes")
+        return b.value
+
+
+class Batch:
+    """Many grids stepped together (fdtd_batch_*)."""
+
+    def __init__(self, hbm_bytes):
+        h = C.c_void_p()
+        check(lib().fdtd_batch_create(hbm_bytes, C.byref(h)), "fdtd_batch_create")
+        self._h = h
+        self._inst = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def add(self, inst):
+        i = C.c_int()
+        check(lib().fdtd_batch_add(self._h, C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c),
+                                   C.byref(i)), "fdtd_batch_add")
+        self._inst[i.value] = inst
+        return i.value
+
+    def remove(self, i):
+        check(lib().fdtd_batch_remove(self._h, i), "fdtd_batch_remove")
+        self._inst.pop(i, None)
+
+    def run(self, steps, stream=None):
+        check(lib().fdtd_batch_run(self._h, steps, stream), "fdtd_batch_run")
+
+    def status(self, i):
+        d, t = C.c_int(), C.c_int()
+        check(lib().fdtd_batch_status(self._h, i, C.byref(d), C.byref(t)), "fdtd_batch_status")
+        return d.value, t.value
+
+    def probes(self, i):
+        inst = self._inst[i]
+        rows = inst.steps
+        out = np.zeros((max(rows, 1), max(inst.nprobe, 1)), np.float64)
+        got = lib().fdtd_batch_probe_read(self._h, i, _ptr(out), rows)
+        if got < 0:
+            check(got, "fdtd_batch_probe_read")
+        return out[:got, :inst.nprobe]
+
+    def get(self, i, field, level=0):
+        fid = FIELDS.index(field) if isinstance(field, str) else field
+        nx, ny, nz = self._inst[i].shape
+        out = np.empty(nx * ny * nz, np.float32)
+        check(lib().fdtd_batch_get_field(self._h, i, fid, level, _ptr(out)), "fdtd_batch_get_field")
+        return out.reshape(nz, ny, nx)
+
+    def set_shape(self, tile_y=0, kchunk=0, graph_steps=0):
+        check(lib().fdtd_batch_set_shape(self._h, tile_y, kchunk, graph_steps), "fdtd_batch_set_shape")
+
+    def autotune(self, trial_steps=8, stream=None):
+        ty, kc = C.c_int(), C.c_int()
+        check(lib().fdtd_batch_autotune(self._h, trial_steps, stream, C.byref(ty), C.byref(kc)), "autotune")
+        return ty.value, kc.value
+
+    def info(self):
+        inf = FdtdBatchInfo()
+        check(lib().fdtd_batch_info(self._h, C.byref(inf)), "fdtd_batch_info")
+        return {k: (list(getattr(inf, k)) if k in ("grid", "tiles", "cells") else getattr(inf, k))
+                for k, _ in FdtdBatchInfo._fields_}
+
+
+class Server:
+    """Continuous-batching request loop (fdtd_serve_*)."""
+
+    def __init__(self, hbm_bytes, max_slots=64, segment_steps=8, tile_y=0, kchunk=0):
+        cfg = FdtdServeConfig(hbm_bytes, max_slots, segment_steps, tile_y, kchunk)
+        h = C.c_void_p()
+        check(lib().fdtd_serve_create(C.byref(cfg), C.byref(h)), "fdtd_serve_create")
+        self._h = h
+        self._req = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_serve_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def submit(self, inst):
+        r = C.c_longlong()
+        check(lib().fdtd_serve_submit(self._h, C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c),
+                                      C.byref(r)), "fdtd_serve_submit")
+        self._req[r.value] = (inst.steps, inst.nprobe)
+        return r.value
+
+    def wait(self, r, timeout=-1.0):
+        rc = lib().fdtd_serve_wait(self._h, r, timeout)
+        if rc < 0:
+            check(rc, "fdtd_serve_wait")
+        return rc == 0
+
+    def result(self, r):
+        """(probe series [steps, nprobe], info dict)."""
+        steps, nprobe = self._req[r]
+        out = np.zeros((max(steps, 1), max(nprobe, 1)), np.float64)
+        info = FdtdRequestInfo()
+        got = lib().fdtd_serve_result(self._h, r, _ptr(out), steps, C.byref(info))
+        if got < 0:
+            check(got, "fdtd_serve_result")
+        return out[:got, :nprobe], {k: getattr(info, k) for k, _ in FdtdRequestInfo._fields_}
+
+    def release(self, r):
+        check(lib().fdtd_serve_release(self._h, r), "fdtd_serve_release")
+        self._req.pop(r, None)
+
+    def drain(self, timeout=-1.0):
+        rc = lib().fdtd_serve_drain(self._h, timeout)
+        if rc < 0:
+            check(rc, "fdtd_serve_drain")
+        return rc == 0
+
+    def stats(self):
+        st = FdtdServeStats()
+        check(lib().fdtd_serve_stats(self._h, C.byref(st)), "fdtd_serve_stats")
+        return {k: getattr(st, k) for k, _ in FdtdServeStats._fields_}

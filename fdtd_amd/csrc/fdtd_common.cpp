@@ -1,0 +1,265 @@
+// Host helpers shared by the single-grid context, the batch and the server:
+// error reporting, 16-bit conversions and the per-step coefficients in every
+// storage form.  One definition of each so that a grid stepped alone and
+// the same grid stepped inside a batch see identical coefficients.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fdtd_internal.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+int fdtd_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int fdtd_hip_fail(hipError_t e, const char *what)
+{
+    return fdtd_fail((int)e, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+extern "C" const char *fdtd_last_error(void) { return g_err.c_str(); }
+
+uint16_t f32_to_f16(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);  // NaN
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); // rounds past 65504: inf
+    if (ax <= 0x33000000u) return (uint16_t)sign;              // at most half the smallest subnormal
+    const int e = (int)(ax >> 23) - 127;
+    const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+    if (e < -14) {  // subnormal half: value = r * 2^-24
+        const int shift = -e - 1;  // 24-bit mantissa (2^e scaled) to units of 2^-24
+        const uint32_t half = 1u << (shift - 1);
+        uint32_t r = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1);
+        if (rem > half || (rem == half && (r & 1u))) r++;
+        return (uint16_t)(sign | r);
+    }
+    uint32_t r = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3ffu);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r++;
+    return (uint16_t)(sign | r);
+}
+
+float f16_to_f32(uint16_t h)
+{
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0) {
+        v = std::ldexp((float)m, -24);
+    } else if (e == 31) {
+        const uint32_t x = 0x7f800000u | (m << 13);
+        memcpy(&v, &x, 4);
+    } else {
+        const uint32_t x = ((e + 112) << 23) | (m << 13);
+        memcpy(&v, &x, 4);
+    }
+    return (h & 0x8000u) ? -v : v;
+}
+
+uint16_t f32_to_bf16(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+    return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+float bf16_to_f32(uint16_t h)
+{
+    const uint32_t x = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+MatCoef fdtd_mat_coef(float dDt, float dEps0, float sg, float es, float ei, float tau, int debye_mode)
+{
+    // Same expressions and precisions as fdtd_calculations.cu:141-155.
+    MatCoef m;
+    m.a = 2.0 * dEps0 * ei * tau;
+    const float b = m.a + 2.0 * dDt * (dEps0 * es + sg * tau);
+    m.c = 1.0 / (b + sg * dDt * dDt);
+    m.d = (debye_mode == FDTD_DEBYE_REF) ? (float)(2.0 * b - sg * dDt * dDt)
+                                         : (float)((double)m.a + (double)b - (double)(sg * dDt * dDt));
+    m.pad = 0.0f;
+    m.g1 = 2.0 * (dDt + tau);
+    m.g2 = 2.0 * dDt + 4.0 * tau;
+    m.g3 = 2.0 * tau;
+    m.k1 = m.k2 = m.k3 = m.k5 = 0.0f;
+    return m;
+}
+
+void fdtd_scaled_coef(MatCoef &m, double dt, double eps0, double sg, double es, double ei, double tau, int debye_mode,
+                      double ratio)
+{
+    // Scaled-unit E coefficients from the same a, b, c, d in double; ratio = sD / sE.
+    const double a = 2.0 * eps0 * ei * tau;
+    const double b = a + 2.0 * dt * (eps0 * es + sg * tau);
+    const double c = 1.0 / (b + sg * dt * dt);
+    const double d = (debye_mode == FDTD_DEBYE_REF ? 2.0 * b : a + b) - sg * dt * dt;
+    m.k1 = (float)(c * d);
+    m.k2 = (float)(c * a);
+    m.k3 = (float)(c * eps0 * ratio * 2.0 * dt);
+    m.k5 = (float)(c * eps0 * ratio * 2.0 * tau);
+}
+
+void fdtd_build_lut(const float *table, int nmat, float dt, float eps0, int debye_mode, int storage, double sE,
+                    double sD, MatCoef *lut)
+{
+    for (int m = 0; m < nmat; m++) {
+        const float *t = &table[4 * m];
+        lut[m] = fdtd_mat_coef(dt, eps0, t[0], t[1], t[2], t[3], debye_mode);
+        if (storage != kStoreF32) fdtd_scaled_coef(lut[m], dt, eps0, t[0], t[1], t[2], t[3], debye_mode, sD / sE);
+    }
+}
+
+void fdtd_curl_coefs(const GridGeom &g, int storage, double sE, double sD, double sH, float ch[3], float cd[3])
+{
+    if (storage == kStoreF32) {
+        // the reference's coefficients, in float (fdtd_calculations.cu:49-75, :92-94)
+        ch[0] = g.dt / (g.mu0 * g.dx);
+        ch[1] = g.dt / (g.mu0 * g.dy);
+        ch[2] = g.dt / (g.mu0 * g.dz);
+        cd[0] = g.dt / g.dx;
+        cd[1] = g.dt / g.dy;
+        cd[2] = g.dt / g.dz;
+        return;
+    }
+    // scaled units: H~ = eta0 H / sH, D~ = D / (eps0 sD), E~ = E / sE, so both
+    // curls carry c0 dt / h (eta0 / mu0 = 1 / (eps0 eta0) = c0)
+    const double c0 = 1.0 / std::sqrt((double)g.mu0 * (double)g.eps0);
+    const double h[3] = {g.dx, g.dy, g.dz};
+    for (int q = 0; q < 3; q++) {
+        ch[q] = (float)(c0 * g.dt / h[q] * sE / sH);
+        cd[q] = (float)(c0 * g.dt / h[q] * sH / sD);
+    }
+}
+
+float fdtd_jz_stored(float jz, int storage, float eps0, double sD)
+{
+    if (storage == kStoreF32) return jz;
+    return (float)((double)jz / ((double)eps0 * sD));
+}
+
+float fdtd_jz_at(const float *jz, int jz_len, int jz_limit, int jz_mode, int n)
+{
+    const int idx = (jz_mode == FDTD_JZ_HOLD3) ? (n / 3) * 3 : n;
+    const int lim = jz_limit > 0 ? (jz_limit < jz_len ? jz_limit : jz_len) : jz_len;
+    return (idx >= 0 && idx < lim) ? jz[idx] : 0.0f;
+}
+
+void fdtd_auto_scales(const float *jz, int jz_len, float eps0, const FdtdExt &e, double *sE, double *sD, double *sH)
+{
+    // 1/32 of the largest per-step D/eps0 kick of the source waveform:
+    // fields near a source then sit at O(10^2-10^3) in storage units, far
+    // below the fp16 overflow (65504) and with about 10^7 of range above the
+    // fp16 subnormals.
+    double peak = 0.0;
+    for (int t = 0; t < jz_len; t++) peak = std::fmax(peak, (double)std::fabs(jz[t]));
+    const double s = peak > 0.0 ? peak / (double)eps0 / 32.0 : 1.0;
+    *sE = e.scale_e > 0.0f ? e.scale_e : s;
+    *sD = e.scale_d > 0.0f ? e.scale_d : s;
+    *sH = e.scale_h > 0.0f ? e.scale_h : s;
+}
+
+double fdtd_field_factor(int storage, float mu0, float eps0, double sE, double sD, double sH, int field)
+{
+    if (storage == kStoreF32) return 1.0;
+    const double eta0 = std::sqrt((double)mu0 / (double)eps0);
+    if (field < 3) return (double)eps0 * sD;
+    if (field < 6) return sH / eta0;
+    return sE;
+}
+
+size_t fdtd_storage_bytes(int storage) { return (storage == kStoreF32 || storage == kStoreF32Fast) ? 4 : 2; }
+
+void fdtd_to_storage(int storage, const float *in, size_t n, double factor, void *out)
+{
+    if (storage == kStoreF32) {
+        memcpy(out, in, n * 4);
+        return;
+    }
+    if (storage == kStoreF32Fast) {
+        float *o = (float *)out;
+        for (size_t t = 0; t < n; t++) o[t] = (float)((double)in[t] / factor);
+        return;
+    }
+    uint16_t *o = (uint16_t *)out;
+    for (size_t t = 0; t < n; t++) {
+        const float v = (float)((double)in[t] / factor);
+        o[t] = storage == kStoreF16 ? f32_to_f16(v) : f32_to_bf16(v);
+    }
+}
+
+void fdtd_from_storage(int storage, const void *in, size_t n, double factor, float *out)
+{
+    if (storage == kStoreF32) {
+        memcpy(out, in, n * 4);
+        return;
+    }
+    if (storage == kStoreF32Fast) {
+        const float *f = (const float *)in;
+        for (size_t t = 0; t < n; t++) out[t] = (float)((double)f[t] * factor);
+        return;
+    }
+    const uint16_t *h = (const uint16_t *)in;
+    for (size_t t = 0; t < n; t++) {
+        const float v = storage == kStoreF16 ? f16_to_f32(h[t]) : bf16_to_f32(h[t]);
+        out[t] = (float)((double)v * factor);
+    }
+}
+
+int fdtd_check_ext(const FdtdExt &e)
+{
+    if (e.debye_mode != FDTD_DEBYE_REF && e.debye_mode != FDTD_DEBYE_FIXED)
+        return fdtd_fail(FDTD_EINVAL, "debye_mode %d", e.debye_mode);
+    if (e.jz_index_mode != FDTD_JZ_STEP && e.jz_index_mode != FDTD_JZ_HOLD3)
+        return fdtd_fail(FDTD_EINVAL, "jz_index_mode %d", e.jz_index_mode);
+    if (e.tile_y != 0 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fdtd_fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
+    if (e.kchunk < 0 || e.jz_limit < 0) return fdtd_fail(FDTD_EINVAL, "negative kchunk/jz_limit");
+    if (e.xcd_remap < 0 || e.xcd_remap > 2) return fdtd_fail(FDTD_EINVAL, "xcd_remap %d", e.xcd_remap);
+    if (e.storage != FDTD_STORE_F32 && e.storage != FDTD_STORE_F16 && e.storage != FDTD_STORE_BF16 &&
+        e.storage != FDTD_STORE_F32_FAST)
+        return fdtd_fail(FDTD_EINVAL, "storage %d", e.storage);
+    if (!(e.scale_e >= 0.0f && e.scale_d >= 0.0f && e.scale_h >= 0.0f)) return fdtd_fail(FDTD_EINVAL, "negative scale");
+    return 0;
+}
+
+int fdtd_check_sources(int nx, int ny, int nz, int nsrc, const int *xyz)
+{
+    for (int s = 0; s < nsrc; s++) {
+        const int x = xyz[3 * s], y = xyz[3 * s + 1], z = xyz[3 * s + 2];
+        if (x < 1 || x >= nx - 1 || y < 1 || y >= ny - 1 || z < 0 || z >= nz - 1)
+            return fdtd_fail(FDTD_ERANGE, "source %d at (%d,%d,%d) outside the Dz update range", s, x, y, z);
+    }
+    return 0;
+}
+
+int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes)
+{
+    for (int p = 0; p < nprobe; p++) {
+        const int *q = probes + 4 * p;
+        if (q[0] < 0 || q[0] > 8 || q[1] < 0 || q[1] >= nx || q[2] < 0 || q[2] >= ny || q[3] < 0 || q[3] >= nz)
+            return fdtd_fail(FDTD_ERANGE, "probe %d out of range", p);
+    }
+    return 0;
+}

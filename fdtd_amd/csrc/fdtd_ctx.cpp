@@ -11,135 +11,14 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdarg>
-#include <cstdio>
 #include <cstring>
 #include <map>
-#include <string>
 #include <tuple>
 #include <vector>
 
 #include "fdtd_internal.h"
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const char *fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-int hip_fail(hipError_t e, const char *what)
-{
-    return fail((int)e, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
-}
-
-#define HIP_TRY(call)                                    \
-    do {                                                 \
-        hipError_t e_ = (call);                          \
-        if (e_ != hipSuccess) return hip_fail(e_, #call); \
-    } while (0)
-
-// Host float <-> 16-bit conversions, round to nearest even.
-uint16_t f32_to_f16(float f)
-{
-    uint32_t x;
-    memcpy(&x, &f, 4);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t ax = x & 0x7fffffffu;
-    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);  // NaN
-    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); // rounds past 65504: inf
-    if (ax <= 0x33000000u) return (uint16_t)sign;              // at most half the smallest subnormal
-    const int e = (int)(ax >> 23) - 127;
-    const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
-    if (e < -14) {  // subnormal half: value = r * 2^-24
-        const int shift = -e - 1;  // 24-bit mantissa (2^e scaled) to units of 2^-24
-        const uint32_t half = 1u << (shift - 1);
-        uint32_t r = m >> shift;
-        const uint32_t rem = m & ((1u << shift) - 1);
-        if (rem > half || (rem == half && (r & 1u))) r++;
-        return (uint16_t)(sign | r);
-    }
-    uint32_t r = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3ffu);
-    const uint32_t rem = m & 0x1fffu;
-    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r++;
-    return (uint16_t)(sign | r);
-}
-
-float f16_to_f32(uint16_t h)
-{
-    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
-    float v;
-    if (e == 0) {
-        v = std::ldexp((float)m, -24);
-    } else if (e == 31) {
-        const uint32_t x = 0x7f800000u | (m << 13);
-        memcpy(&v, &x, 4);
-    } else {
-        const uint32_t x = ((e + 112) << 23) | (m << 13);
-        memcpy(&v, &x, 4);
-    }
-    return (h & 0x8000u) ? -v : v;
-}
-
-uint16_t f32_to_bf16(float f)
-{
-    uint32_t x;
-    memcpy(&x, &f, 4);
-    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
-    return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
-}
-
-float bf16_to_f32(uint16_t h)
-{
-    const uint32_t x = (uint32_t)h << 16;
-    float f;
-    memcpy(&f, &x, 4);
-    return f;
-}
-
-}  // namespace
-
-MatCoef fdtd_mat_coef(float dDt, float dEps0, float sg, float es, float ei, float tau, int debye_mode)
-{
-    // Same expressions and precisions as fdtd_calculations.cu:141-155.
-    MatCoef m;
-    m.a = 2.0 * dEps0 * ei * tau;
-    const float b = m.a + 2.0 * dDt * (dEps0 * es + sg * tau);
-    m.c = 1.0 / (b + sg * dDt * dDt);
-    m.d = (debye_mode == FDTD_DEBYE_REF) ? (float)(2.0 * b - sg * dDt * dDt)
-                                         : (float)((double)m.a + (double)b - (double)(sg * dDt * dDt));
-    m.pad = 0.0f;
-    m.g1 = 2.0 * (dDt + tau);
-    m.g2 = 2.0 * dDt + 4.0 * tau;
-    m.g3 = 2.0 * tau;
-    m.k1 = m.k2 = m.k3 = m.k5 = 0.0f;
-    return m;
-}
-
-namespace {
-
-// Scaled-unit E coefficients (see MatCoef), from the same a, b, c, d in
-// double; ratio = sD / sE.
-void scaled_coef(MatCoef &m, double dt, double eps0, double sg, double es, double ei, double tau, int debye_mode,
-                 double ratio)
-{
-    const double a = 2.0 * eps0 * ei * tau;
-    const double b = a + 2.0 * dt * (eps0 * es + sg * tau);
-    const double c = 1.0 / (b + sg * dt * dt);
-    const double d = (debye_mode == FDTD_DEBYE_REF ? 2.0 * b : a + b) - sg * dt * dt;
-    m.k1 = (float)(c * d);
-    m.k2 = (float)(c * a);
-    m.k3 = (float)(c * eps0 * ratio * 2.0 * dt);
-    m.k5 = (float)(c * eps0 * ratio * 2.0 * tau);
-}
 
 // Extra elements between consecutive field arrays in the blob: staggers the
 // arrays' starts across HBM channels (a no-compute 15-read/9-write stream
@@ -208,48 +87,7 @@ char *field_dev(fdtd_ctx *c, int field, int level)
 // stored value -> SI value of a field
 double field_factor(const fdtd_ctx *c, int field)
 {
-    if (c->storage == kStoreF32) return 1.0;
-    const double eta0 = std::sqrt((double)c->mu0 / (double)c->eps0);
-    if (field < 3) return (double)c->eps0 * c->sD;
-    if (field < 6) return c->sH / eta0;
-    return c->sE;
-}
-
-void to_storage(const fdtd_ctx *c, const float *in, size_t n, double factor, std::vector<char> &out)
-{
-    out.resize(n * c->esz);
-    if (c->storage == kStoreF32) {
-        memcpy(out.data(), in, n * 4);
-        return;
-    }
-    if (c->storage == kStoreF32Fast) {
-        float *o = (float *)out.data();
-        for (size_t t = 0; t < n; t++) o[t] = (float)((double)in[t] / factor);
-        return;
-    }
-    uint16_t *o = (uint16_t *)out.data();
-    for (size_t t = 0; t < n; t++) {
-        const float v = (float)((double)in[t] / factor);
-        o[t] = c->storage == kStoreF16 ? f32_to_f16(v) : f32_to_bf16(v);
-    }
-}
-
-void from_storage(const fdtd_ctx *c, const char *in, size_t n, double factor, float *out)
-{
-    if (c->storage == kStoreF32) {
-        memcpy(out, in, n * 4);
-        return;
-    }
-    if (c->storage == kStoreF32Fast) {
-        const float *f = (const float *)in;
-        for (size_t t = 0; t < n; t++) out[t] = (float)((double)f[t] * factor);
-        return;
-    }
-    const uint16_t *h = (const uint16_t *)in;
-    for (size_t t = 0; t < n; t++) {
-        const float v = c->storage == kStoreF16 ? f16_to_f32(h[t]) : bf16_to_f32(h[t]);
-        out[t] = (float)((double)v * factor);
-    }
+    return fdtd_field_factor(c->storage, c->mu0, c->eps0, c->sE, c->sD, c->sH, field);
 }
 
 int upload_ids(fdtd_ctx *c)
@@ -265,20 +103,12 @@ int upload_ids(fdtd_ctx *c)
     return 0;
 }
 
-// 16-bit scales: explicit ones from FdtdExt, else 1/32 of the largest
-// per-step D/eps0 kick of the source waveform: fields near a source then sit
-// at O(10^2-10^3) in storage units, far below the fp16 overflow (65504) and
-// with about 10^7 of range above the fp16 subnormals.  Fixed from the first
-// step on.
+// 16-bit scales: explicit ones from FdtdExt, else derived from the source
+// waveform (fdtd_auto_scales); fixed from the first step on.
 void resolve_scales(fdtd_ctx *c)
 {
     if (c->scales_set) return;
-    double peak = 0.0;
-    for (float v : c->jz) peak = std::max(peak, (double)std::fabs(v));
-    const double s = peak > 0.0 ? peak / (double)c->eps0 / 32.0 : 1.0;
-    c->sE = c->ext.scale_e > 0.0f ? c->ext.scale_e : s;
-    c->sD = c->ext.scale_d > 0.0f ? c->ext.scale_d : s;
-    c->sH = c->ext.scale_h > 0.0f ? c->ext.scale_h : s;
+    fdtd_auto_scales(c->jz.data(), (int)c->jz.size(), c->eps0, c->ext, &c->sE, &c->sD, &c->sH);
     c->scales_set = true;
     c->lut_dirty = true;
 }
@@ -287,20 +117,13 @@ int upload_lut(fdtd_ctx *c)
 {
     if (!c->lut_dirty) return 0;
     std::vector<MatCoef> lut(c->nmat);
-    for (int m = 0; m < c->nmat; m++) {
-        const float *t = &c->table[4 * m];
-        lut[m] = fdtd_mat_coef(c->dt, c->eps0, t[0], t[1], t[2], t[3], c->ext.debye_mode);
-        if (c->storage != kStoreF32)
-            scaled_coef(lut[m], c->dt, c->eps0, t[0], t[1], t[2], t[3], c->ext.debye_mode, c->sD / c->sE);
-    }
+    fdtd_build_lut(c->table.data(), c->nmat, c->dt, c->eps0, c->ext.debye_mode, c->storage, c->sE, c->sD, lut.data());
     HIP_TRY(hipMemcpy(c->lut, lut.data(), sizeof(MatCoef) * c->nmat, hipMemcpyHostToDevice));
     c->lut_dirty = false;
     return 0;
 }
 
 }  // namespace
-
-extern "C" const char *fdtd_last_error(void) { return g_err.c_str(); }
 
 extern "C" int fdtd_ext_defaults(FdtdExt *ext)
 {
@@ -318,26 +141,18 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
 
 extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx **out)
 {
-    if (!p || !out) return fail(FDTD_EINVAL, "null argument");
+    if (!p || !out) return fdtd_fail(FDTD_EINVAL, "null argument");
     *out = nullptr;
-    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fdtd_fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
     const long long N = (long long)p->nx * p->ny * p->nz;
     // The fused kernel addresses each field array with 32-bit byte offsets
     // that span the array and its guards.
     if (N + 2 * fused_guard(p->nx, (long long)p->nx * p->ny) >= (1LL << 30))
-        return fail(FDTD_ERANGE, "grid of %lld cells: too large for one context", N);
+        return fdtd_fail(FDTD_ERANGE, "grid of %lld cells: too large for one context", N);
     FdtdExt e;
     fdtd_ext_defaults(&e);
     if (ext) e = *ext;
-    if (e.debye_mode != FDTD_DEBYE_REF && e.debye_mode != FDTD_DEBYE_FIXED) return fail(FDTD_EINVAL, "debye_mode %d", e.debye_mode);
-    if (e.jz_index_mode != FDTD_JZ_STEP && e.jz_index_mode != FDTD_JZ_HOLD3) return fail(FDTD_EINVAL, "jz_index_mode %d", e.jz_index_mode);
-    if (e.tile_y != 0 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
-    if (e.kchunk < 0 || e.jz_limit < 0) return fail(FDTD_EINVAL, "negative kchunk/jz_limit");
-    if (e.xcd_remap < 0 || e.xcd_remap > 2) return fail(FDTD_EINVAL, "xcd_remap %d", e.xcd_remap);
-    if (e.storage != FDTD_STORE_F32 && e.storage != FDTD_STORE_F16 && e.storage != FDTD_STORE_BF16 &&
-        e.storage != FDTD_STORE_F32_FAST)
-        return fail(FDTD_EINVAL, "storage %d", e.storage);
-    if (!(e.scale_e >= 0.0f && e.scale_d >= 0.0f && e.scale_h >= 0.0f)) return fail(FDTD_EINVAL, "negative scale");
+    if (int rc = fdtd_check_ext(e)) return rc;
 
     fdtd_ctx *c = new fdtd_ctx();
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
@@ -347,7 +162,7 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->dt = p->dt; c->dx = p->dx; c->dy = p->dy; c->dz = p->dz; c->mu0 = p->mu0; c->eps0 = p->eps0;
     c->ext = e;
     c->storage = e.storage;
-    c->esz = (e.storage == FDTD_STORE_F32 || e.storage == FDTD_STORE_F32_FAST) ? 4 : 2;
+    c->esz = fdtd_storage_bytes(e.storage);
     // Defaults from launch-shape sweeps on MI355X (256^3 and 320x320x880
     // Debye phantoms): 64x4 tiles marching 8 planes, identity block order.
     c->ty = e.tile_y ? e.tile_y : 4;
@@ -360,7 +175,7 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->guard = fused_guard(c->pitch, c->plane);
     const size_t blob_elems = nf * (c->N + kArraySkew) + 2 * (size_t)c->guard;
     hipError_t he = hipMalloc(&c->blob, blob_elems * c->esz);
-    if (he != hipSuccess) { delete c; return hip_fail(he, "hipMalloc(fields)"); }
+    if (he != hipSuccess) { delete c; return fdtd_hip_fail(he, "hipMalloc(fields)"); }
     char *q = c->blob + c->guard * c->esz;
     const size_t stride = (c->N + kArraySkew) * c->esz;
     for (int s = 0; s < 3; s++)
@@ -384,7 +199,7 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
         }
     }
     if (he != hipSuccess) {
-        int rc = hip_fail(he, "fdtd_ctx_create");
+        int rc = fdtd_hip_fail(he, "fdtd_ctx_create");
         fdtd_ctx_destroy(c);
         return rc;
     }
@@ -417,10 +232,10 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
 
 extern "C" int fdtd_ctx_set_materials(fdtd_ctx *c, const unsigned char *ids, int nmat, const float *table)
 {
-    if (!c || !ids || !table) return fail(FDTD_EINVAL, "null argument");
-    if (nmat < 1 || nmat > kMaxMaterials) return fail(FDTD_ERANGE, "nmat %d not in [1,%d]", nmat, kMaxMaterials);
+    if (!c || !ids || !table) return fdtd_fail(FDTD_EINVAL, "null argument");
+    if (nmat < 1 || nmat > kMaxMaterials) return fdtd_fail(FDTD_ERANGE, "nmat %d not in [1,%d]", nmat, kMaxMaterials);
     for (size_t t = 0; t < c->N; t++)
-        if (ids[t] >= nmat) return fail(FDTD_ERANGE, "material id %d at cell %zu >= nmat %d", ids[t], t, nmat);
+        if (ids[t] >= nmat) return fdtd_fail(FDTD_ERANGE, "material id %d at cell %zu >= nmat %d", ids[t], t, nmat);
     c->table.assign(table, table + 4 * nmat);
     c->nmat = nmat;
     c->lut_dirty = true;
@@ -431,7 +246,7 @@ extern "C" int fdtd_ctx_set_materials(fdtd_ctx *c, const unsigned char *ids, int
 extern "C" int fdtd_ctx_set_material_arrays(fdtd_ctx *c, const float *sigma, const float *epsS, const float *epsI,
                                             const float *tauD)
 {
-    if (!c || !sigma || !epsS || !epsI || !tauD) return fail(FDTD_EINVAL, "null argument");
+    if (!c || !sigma || !epsS || !epsI || !tauD) return fdtd_fail(FDTD_EINVAL, "null argument");
     std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, int> seen;
     std::vector<float> table;
     std::vector<unsigned char> ids(c->N);
@@ -443,7 +258,7 @@ extern "C" int fdtd_ctx_set_material_arrays(fdtd_ctx *c, const float *sigma, con
         int id;
         if (it == seen.end()) {
             id = (int)seen.size();
-            if (id >= kMaxMaterials) return fail(FDTD_ERANGE, "more than %d distinct materials", kMaxMaterials);
+            if (id >= kMaxMaterials) return fdtd_fail(FDTD_ERANGE, "more than %d distinct materials", kMaxMaterials);
             seen.emplace(key, id);
             table.insert(table.end(), {sigma[t], epsS[t], epsI[t], tauD[t]});
         } else {
@@ -457,12 +272,12 @@ extern "C" int fdtd_ctx_set_material_arrays(fdtd_ctx *c, const float *sigma, con
 extern "C" int fdtd_ctx_set_rp(fdtd_ctx *c, const float *rpx0, const float *rpy0, const float *rpz0,
                                const float *rpxEnd, const float *rpyEnd, const float *rpzEnd)
 {
-    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     const float *src[6] = {rpx0, rpy0, rpz0, rpxEnd, rpyEnd, rpzEnd};
     const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
                            (size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2};
     for (int r = 0; r < 6; r++) {
-        if (!src[r]) return fail(FDTD_EINVAL, "null rp plane %d", r);
+        if (!src[r]) return fdtd_fail(FDTD_EINVAL, "null rp plane %d", r);
         HIP_TRY(hipMemcpy(c->rp[r], src[r], rps[r] * sizeof(float), hipMemcpyHostToDevice));
     }
     return 0;
@@ -470,12 +285,8 @@ extern "C" int fdtd_ctx_set_rp(fdtd_ctx *c, const float *rpx0, const float *rpy0
 
 extern "C" int fdtd_ctx_set_sources(fdtd_ctx *c, int nsrc, const int *xyz, int jz_len, const float *jz)
 {
-    if (!c || nsrc < 0 || jz_len < 0 || (nsrc > 0 && !xyz) || (jz_len > 0 && !jz)) return fail(FDTD_EINVAL, "bad sources");
-    for (int s = 0; s < nsrc; s++) {
-        const int x = xyz[3 * s], y = xyz[3 * s + 1], z = xyz[3 * s + 2];
-        if (x < 1 || x >= c->nx - 1 || y < 1 || y >= c->ny - 1 || z < 0 || z >= c->nz - 1)
-            return fail(FDTD_ERANGE, "source %d at (%d,%d,%d) outside the Dz update range", s, x, y, z);
-    }
+    if (!c || nsrc < 0 || jz_len < 0 || (nsrc > 0 && !xyz) || (jz_len > 0 && !jz)) return fdtd_fail(FDTD_EINVAL, "bad sources");
+    if (int rc = fdtd_check_sources(c->nx, c->ny, c->nz, nsrc, xyz)) return rc;
     if (c->n == 0) c->scales_set = false;  // auto scales follow the waveform until the first step
     if (c->ids_host.empty()) c->ids_host.assign(c->N, 0);
     c->src.assign(xyz, xyz + 3 * nsrc);
@@ -485,12 +296,8 @@ extern "C" int fdtd_ctx_set_sources(fdtd_ctx *c, int nsrc, const int *xyz, int j
 
 extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, int capacity)
 {
-    if (!c || nprobe < 0 || capacity < 0 || (nprobe > 0 && !probes)) return fail(FDTD_EINVAL, "bad probes");
-    for (int p = 0; p < nprobe; p++) {
-        const int *q = probes + 4 * p;
-        if (q[0] < 0 || q[0] > 8 || q[1] < 0 || q[1] >= c->nx || q[2] < 0 || q[2] >= c->ny || q[3] < 0 || q[3] >= c->nz)
-            return fail(FDTD_ERANGE, "probe %d out of range", p);
-    }
+    if (!c || nprobe < 0 || capacity < 0 || (nprobe > 0 && !probes)) return fdtd_fail(FDTD_EINVAL, "bad probes");
+    if (int rc = fdtd_check_probes(c->nx, c->ny, c->nz, nprobe, probes)) return rc;
     (void)hipFree(c->probes);
     (void)hipFree(c->probe_out);
     c->probes = nullptr;
@@ -508,8 +315,8 @@ extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, i
 
 extern "C" int fdtd_ctx_set_slices(fdtd_ctx *c, int k, int interval, int capacity)
 {
-    if (!c || interval < 0 || capacity < 0) return fail(FDTD_EINVAL, "bad slices");
-    if (interval > 0 && (k < 0 || k >= c->nz)) return fail(FDTD_ERANGE, "slice plane %d out of range", k);
+    if (!c || interval < 0 || capacity < 0) return fdtd_fail(FDTD_EINVAL, "bad slices");
+    if (interval > 0 && (k < 0 || k >= c->nz)) return fdtd_fail(FDTD_ERANGE, "slice plane %d out of range", k);
     if (c->side) HIP_TRY(hipStreamSynchronize(c->side));
     (void)hipFree(c->slice_stage);
     if (c->slice_host) (void)hipHostFree(c->slice_host);
@@ -531,10 +338,10 @@ extern "C" int fdtd_ctx_set_slices(fdtd_ctx *c, int k, int interval, int capacit
 
 extern "C" int fdtd_ctx_set_field(fdtd_ctx *c, int field, int level, const float *host)
 {
-    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fdtd_fail(FDTD_EINVAL, "bad field/level");
     resolve_scales(c);
-    std::vector<char> buf;
-    to_storage(c, host, c->N, field_factor(c, field), buf);
+    std::vector<char> buf(c->N * c->esz);
+    fdtd_to_storage(c->storage, host, c->N, field_factor(c, field), buf.data());
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(field_dev(c, field, level), buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
     return 0;
@@ -542,25 +349,25 @@ extern "C" int fdtd_ctx_set_field(fdtd_ctx *c, int field, int level, const float
 
 extern "C" int fdtd_ctx_get_field(fdtd_ctx *c, int field, int level, float *host)
 {
-    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fdtd_fail(FDTD_EINVAL, "bad field/level");
     resolve_scales(c);
     std::vector<char> buf(c->N * c->esz);
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(buf.data(), field_dev(c, field, level), c->N * c->esz, hipMemcpyDeviceToHost));
-    from_storage(c, buf.data(), c->N, field_factor(c, field), host);
+    fdtd_from_storage(c->storage, buf.data(), c->N, field_factor(c, field), host);
     return 0;
 }
 
 extern "C" int fdtd_ctx_field_ptr(fdtd_ctx *c, int field, int level, void **dev)
 {
-    if (!c || !dev || field < 0 || field > 8 || level < 0 || level > 2) return fail(FDTD_EINVAL, "bad field/level");
+    if (!c || !dev || field < 0 || field > 8 || level < 0 || level > 2) return fdtd_fail(FDTD_EINVAL, "bad field/level");
     *dev = field_dev(c, field, level);
     return 0;
 }
 
 extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
 {
-    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     resolve_scales(c);
     int rc = upload_lut(c);
     if (rc) return rc;
@@ -580,32 +387,17 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.ids = c->ids;
     a.lut = c->lut;
     a.nmat = c->nmat;
-    int idx = (c->ext.jz_index_mode == FDTD_JZ_HOLD3) ? (n / 3) * 3 : n;
-    const int lim = c->ext.jz_limit > 0 ? std::min<int>(c->ext.jz_limit, (int)c->jz.size()) : (int)c->jz.size();
-    const float jz = (idx >= 0 && idx < lim) ? c->jz[idx] : 0.0f;
+    const float jz = fdtd_jz_at(c->jz.data(), (int)c->jz.size(), c->ext.jz_limit, c->ext.jz_index_mode, n);
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
     a.pitch = c->pitch;
     a.plane = c->plane;
-    if (c->storage == kStoreF32) {
-        // the reference's coefficients, in float (fdtd_calculations.cu:49-75, :92-94)
-        a.jz = jz;
-        a.chx = c->dt / (c->mu0 * c->dx);
-        a.chy = c->dt / (c->mu0 * c->dy);
-        a.chz = c->dt / (c->mu0 * c->dz);
-        a.cdx = c->dt / c->dx;
-        a.cdy = c->dt / c->dy;
-        a.cdz = c->dt / c->dz;
-    } else {
-        // scaled units: H~ = eta0 H / sH, D~ = D / (eps0 sD), E~ = E / sE, so
-        // both curls carry c0 dt / h (eta0 / mu0 = 1 / (eps0 eta0) = c0)
-        const double c0 = 1.0 / std::sqrt((double)c->mu0 * (double)c->eps0);
-        a.jz = (float)((double)jz / ((double)c->eps0 * c->sD));
-        a.chx = (float)(c0 * c->dt / c->dx * c->sE / c->sH);
-        a.chy = (float)(c0 * c->dt / c->dy * c->sE / c->sH);
-        a.chz = (float)(c0 * c->dt / c->dz * c->sE / c->sH);
-        a.cdx = (float)(c0 * c->dt / c->dx * c->sH / c->sD);
-        a.cdy = (float)(c0 * c->dt / c->dy * c->sH / c->sD);
-        a.cdz = (float)(c0 * c->dt / c->dz * c->sH / c->sD);
+    a.jz = fdtd_jz_stored(jz, c->storage, c->eps0, c->sD);
+    {
+        const GridGeom g{c->dt, c->dx, c->dy, c->dz, c->mu0, c->eps0};
+        float ch[3], cd[3];
+        fdtd_curl_coefs(g, c->storage, c->sE, c->sD, c->sH, ch, cd);
+        a.chx = ch[0]; a.chy = ch[1]; a.chz = ch[2];
+        a.cdx = cd[0]; a.cdy = cd[1]; a.cdz = cd[2];
     }
     a.kchunk = c->kchunk;
     a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
@@ -614,7 +406,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     const bool timed = c->tused < c->tcap;
     if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
     hipError_t e = launch_fused_step(a, c->ty, c->storage, stream);
-    if (e != hipSuccess) return hip_fail(e, "fused_step");
+    if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step");
     if (timed) {
         HIP_TRY(hipEventRecord(c->tev[2 * c->tused + 1], stream));
         c->tused++;
@@ -629,7 +421,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     m.dt = c->dt; m.hx = c->dx; m.hy = c->dy; m.hz = c->dz; m.mu0 = c->mu0; m.eps0 = c->eps0;
     mur_segments(c->nx, c->ny, c->nz, m.seg_start);
     e = launch_mur(m, c->storage, stream);
-    if (e != hipSuccess) return hip_fail(e, "mur_faces");
+    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces");
 
     c->n = n + 1;
     c->hcur = hn;
@@ -647,7 +439,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
         p.out = c->probe_out;
         p.row = c->probe_rows++;
         e = launch_probes(p, c->storage, stream);
-        if (e != hipSuccess) return hip_fail(e, "probe_cells");
+        if (e != hipSuccess) return fdtd_hip_fail(e, "probe_cells");
     }
     if (c->slice_interval > 0 && c->n % c->slice_interval == 0 && c->slice_count < c->slice_cap) {
         const size_t pb = (size_t)c->plane * c->esz;
@@ -667,7 +459,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
 
 extern "C" int fdtd_run(fdtd_ctx *c, int steps, hipStream_t stream)
 {
-    if (!c || steps < 0) return fail(FDTD_EINVAL, "bad run");
+    if (!c || steps < 0) return fdtd_fail(FDTD_EINVAL, "bad run");
     for (int s = 0; s < steps; s++) {
         int rc = fdtd_step_fused(c, stream);
         if (rc) return rc;
@@ -679,7 +471,7 @@ extern "C" int fdtd_ctx_steps_done(fdtd_ctx *c) { return c ? c->n : FDTD_EINVAL;
 
 extern "C" int fdtd_probe_read(fdtd_ctx *c, double *host, int max_rows)
 {
-    if (!c || (!host && max_rows > 0)) return fail(FDTD_EINVAL, "bad probe read");
+    if (!c || (!host && max_rows > 0)) return fdtd_fail(FDTD_EINVAL, "bad probe read");
     HIP_TRY(hipDeviceSynchronize());
     const int rows = std::min(max_rows, c->probe_rows);
     if (rows > 0 && c->nprobe > 0)
@@ -689,7 +481,7 @@ extern "C" int fdtd_probe_read(fdtd_ctx *c, double *host, int max_rows)
 
 extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_slices)
 {
-    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     if (c->side) HIP_TRY(hipStreamSynchronize(c->side));
     const int cnt = std::min(max_slices, c->slice_count);
     const size_t pl = (size_t)c->plane;
@@ -697,7 +489,7 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
         if (steps) steps[s] = c->slice_steps[s];
         if (data)
             for (int f = 0; f < 9; f++)
-                from_storage(c, c->slice_host + ((size_t)s * 9 + f) * pl * c->esz, pl, field_factor(c, f),
+                fdtd_from_storage(c->storage, c->slice_host + ((size_t)s * 9 + f) * pl * c->esz, pl, field_factor(c, f),
                              data + ((size_t)s * 9 + f) * pl);
     }
     return cnt;
@@ -705,7 +497,7 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
 
 extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int *kchunk, int *blocks)
 {
-    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     if (tile_x) *tile_x = 64;
     if (tile_y) *tile_y = c->ty;
     if (kchunk) *kchunk = c->kchunk;
@@ -715,7 +507,7 @@ extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int 
 
 extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity)
 {
-    if (!c || capacity < 0) return fail(FDTD_EINVAL, "bad timing capacity");
+    if (!c || capacity < 0) return fdtd_fail(FDTD_EINVAL, "bad timing capacity");
     HIP_TRY(hipDeviceSynchronize());
     for (auto ev : c->tev) (void)hipEventDestroy(ev);
     c->tev.assign(2 * (size_t)capacity, nullptr);
@@ -727,7 +519,7 @@ extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity)
 
 extern "C" int fdtd_ctx_kernel_time(fdtd_ctx *c, double *ms_total, int *launches)
 {
-    if (!c || !ms_total) return fail(FDTD_EINVAL, "null argument");
+    if (!c || !ms_total) return fdtd_fail(FDTD_EINVAL, "null argument");
     double acc = 0.0;
     for (int t = 0; t < c->tused; t++) {
         HIP_TRY(hipEventSynchronize(c->tev[2 * t + 1]));
@@ -742,7 +534,7 @@ extern "C" int fdtd_ctx_kernel_time(fdtd_ctx *c, double *ms_total, int *launches
 
 extern "C" int fdtd_ctx_scales(fdtd_ctx *c, double *scale_e, double *scale_d, double *scale_h)
 {
-    if (!c) return fail(FDTD_EINVAL, "null context");
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     resolve_scales(c);
     if (scale_e) *scale_e = c->sE;
     if (scale_d) *scale_d = c->sD;

@@ -91,3 +91,110 @@ hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
 void mur_segments(int nx, int ny, int nz, int *seg_start);
+
+// ---- batches of independent grids -------------------------------------
+//
+// A batch advances many grids with one persistent fused launch per storage
+// class and step, one Mur launch and one probe launch.  The launches read
+// everything that changes from device memory (the instance descriptors, the
+// per-class work tables and a batch step counter), so a captured sequence of
+// steps replays unchanged while grids join and leave between replays.
+
+constexpr int kBatchMaxInst = 512;        // descriptors per batch
+constexpr int kBatchMaxProbes = 64;       // probes per instance
+constexpr int kBatchClasses = 4;          // one per Storage value
+constexpr int kMurBlock = 256;            // Mur face cells per block
+
+// Device descriptor of one grid of a batch.  Instance step n (0-based) runs
+// at batch step start + n for n < nsteps; outside that window the instance
+// is left untouched.
+struct BatchInst {
+    void *E[3][3];                 // [ring slot][component]
+    void *D[3][3];
+    void *H[2][3];                 // [buffer][component]; buffer n % 2 holds H^{n-1/2}
+    const uint8_t *ids;            // with fused_guard() bytes before and after
+    const MatCoef *lut;
+    const float *jz;               // per-step source values in storage units
+    const float *rp[6];            // rpx0 rpy0 rpz0 rpxEnd rpyEnd rpzEnd
+    const int *probes;             // nprobe x (field, i, j, k)
+    double *probe_out;             // [step][probe], nsteps rows
+    double factor[9];              // stored value -> SI, per field
+    long long plane;
+    int nmat, jz_len, jz_hold3, start, nsteps, storage;
+    int nx, ny, nz, pitch;
+    unsigned guard;
+    float ch[3], cd[3];
+    int kchunk, tiles_x, tiles_y, tiles_z;
+    int nprobe, probe_rows;        // probe_out holds probe_rows rows
+    float dt, hx, hy, hz, mu0, eps0;
+    int mur_seg[13];
+};
+
+// Work list of one storage class: instance indices and prefix sums of their
+// fused tiles.
+struct BatchClass {
+    int n, total;
+    int idx[kBatchMaxInst];
+    int start[kBatchMaxInst + 1];
+};
+
+// Work lists shared by all classes: Mur blocks per instance (prefix sums in
+// units of kMurBlock face cells) and the flattened probe records.
+struct BatchShared {
+    int step;                      // batch step counter (advanced on the device)
+    int nmur, mur_total;
+    int mur_idx[kBatchMaxInst];
+    int mur_start[kBatchMaxInst + 1];
+    int nrec;
+    int rec[kBatchMaxInst * kBatchMaxProbes][2];   // (instance, probe)
+};
+
+struct BatchLaunch {
+    const BatchInst *inst;
+    const BatchClass *cls;         // the launch's class
+    BatchShared *sh;
+    int off;                       // step offset from sh->step
+};
+
+// LDS bytes of the batch's fused kernel (coefficient table of kMaxMaterials).
+size_t batch_fused_lds();
+hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int storage, int grid, hipStream_t s);
+hipError_t launch_mur_batch(const BatchLaunch &l, int grid, hipStream_t s);
+hipError_t launch_probe_batch(const BatchLaunch &l, int grid, hipStream_t s);
+hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s);
+// Resident blocks of the fused batch kernel per CU.
+int batch_fused_occupancy(int ty, int storage);
+
+// ---- shared host helpers (fdtd_common.cpp) ------------------------------
+
+struct GridGeom {
+    float dt, dx, dy, dz, mu0, eps0;
+};
+
+int fdtd_fail(int code, const char *fmt, ...);
+int fdtd_hip_fail(hipError_t e, const char *what);
+uint16_t f32_to_f16(float f);
+float f16_to_f32(uint16_t h);
+uint16_t f32_to_bf16(float f);
+float bf16_to_f32(uint16_t h);
+void fdtd_scaled_coef(MatCoef &m, double dt, double eps0, double sg, double es, double ei, double tau, int debye_mode,
+                      double ratio);
+void fdtd_build_lut(const float *table, int nmat, float dt, float eps0, int debye_mode, int storage, double sE,
+                    double sD, MatCoef *lut);
+void fdtd_curl_coefs(const GridGeom &g, int storage, double sE, double sD, double sH, float ch[3], float cd[3]);
+float fdtd_jz_stored(float jz, int storage, float eps0, double sD);
+float fdtd_jz_at(const float *jz, int jz_len, int jz_limit, int jz_mode, int n);
+void fdtd_auto_scales(const float *jz, int jz_len, float eps0, const FdtdExt &e, double *sE, double *sD, double *sH);
+double fdtd_field_factor(int storage, float mu0, float eps0, double sE, double sD, double sH, int field);
+size_t fdtd_storage_bytes(int storage);
+void fdtd_to_storage(int storage, const float *in, size_t n, double factor, void *out);
+void fdtd_from_storage(int storage, const void *in, size_t n, double factor, float *out);
+int fdtd_check_ext(const FdtdExt &e);
+int fdtd_check_sources(int nx, int ny, int nz, int nsrc, const int *xyz);
+int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes);
+
+#define HIP_TRY(call)                                              \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return fdtd_hip_fail(e_, #call);     \
+    } while (0)

@@ -149,8 +149,38 @@ __device__ inline float e_update_scaled(float k1, float k2, float k3, float k5, 
 // -y halo row, wave TY-1 loads the +y E row, and wave 1 gathers the -x/+x
 // E columns and updates the -x halo H column (lane l: side l/32, row l%32).
 // Each plane's global loads are issued one plane ahead of their use.
+// The per-material coefficient table lives in dynamic LDS: exact form g1 g2
+// g3 [nmat] doubles then a c d [nmat] floats; scaled form k1 k2 k3 k5 floats.
+__device__ __forceinline__ double *lut_lds()
+{
+    extern __shared__ double sLut[];
+    return sLut;
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void stage_lut(const MatCoef *lut, int nmat, int tid, int nthr)
+{
+    double *s = lut_lds();
+    double *lG1 = s, *lG2 = s + nmat, *lG3 = s + 2 * nmat;
+    float *lA = (float *)(s + 3 * nmat), *lC = lA + nmat, *lD = lA + 2 * nmat;
+    float *lK1 = (float *)s, *lK2 = lK1 + nmat, *lK3 = lK1 + 2 * nmat, *lK5 = lK1 + 3 * nmat;
+    for (int m = tid; m < nmat; m += nthr) {
+        const MatCoef mc = lut[m];
+        if constexpr (EXACT) {
+            lA[m] = mc.a; lC[m] = mc.c; lD[m] = mc.d;
+            lG1[m] = mc.g1; lG2[m] = mc.g2; lG3[m] = mc.g3;
+        } else {
+            lK1[m] = mc.k1; lK2[m] = mc.k2; lK3[m] = mc.k3; lK5[m] = mc.k5;
+        }
+    }
+}
+
+// One 64 x TY x kchunk tile (column tile (bx, by), z-chunk bz) of one step.
+// The coefficient table must be staged; the first LDS read of it follows a
+// barrier inside.  The tile's own LDS writes start at once, so a caller that
+// runs several tiles per block puts a barrier between them.
 template <int TY, typename T, bool EXACT>
-__global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
+__device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, int bz)
 {
     static_assert(TY >= 2 && TY <= 8, "tile rows");
     static_assert(!EXACT || std::is_same<T, float>::value, "the exact form stores fp32");
@@ -162,15 +192,8 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     __shared__ float sE[2][3][TY + 2][TX + 2];
     __shared__ float sH[3][TY + 1][TX + 1];
     __shared__ float sJunk[2][TX];
-    extern __shared__ double sLut[];  // exact: g1 g2 g3 [nmat] doubles, a c d floats; scaled: k1 k2 k3 k5 floats
 
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
-    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
-    const int bx = b % A.tiles_x;
-    const int by = (b / A.tiles_x) % A.tiles_y;
-    const int bz = b / (A.tiles_x * A.tiles_y);
-
+    const int tx = threadIdx.x, ty = threadIdx.y;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int i0 = bx * TX, j0 = by * TY;
     const int k0 = bz * A.kchunk;
@@ -179,18 +202,10 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     const bool own = (i < nx) && (j < ny);
 
     const int nmat = A.nmat;
+    double *sLut = lut_lds();
     double *lG1 = sLut, *lG2 = sLut + nmat, *lG3 = sLut + 2 * nmat;
     float *lA = (float *)(sLut + 3 * nmat), *lC = lA + nmat, *lD = lA + 2 * nmat;
     float *lK1 = (float *)sLut, *lK2 = lK1 + nmat, *lK3 = lK1 + 2 * nmat, *lK5 = lK1 + 3 * nmat;
-    for (int m = tid; m < nmat; m += TX * TY) {
-        const MatCoef mc = A.lut[m];
-        if constexpr (exact) {
-            lA[m] = mc.a; lC[m] = mc.c; lD[m] = mc.d;
-            lG1[m] = mc.g1; lG2[m] = mc.g2; lG3[m] = mc.g3;
-        } else {
-            lK1[m] = mc.k1; lK2[m] = mc.k2; lK3[m] = mc.k3; lK5[m] = mc.k5;
-        }
-    }
 
     // Origin-shifted bases (guard elements before each array's origin).
     const uint32_t gb = A.guard * ES;
@@ -384,6 +399,89 @@ __global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
     }
 }
 
+// One grid, one step: block b runs tile xcd_logical(b).
+template <int TY, typename T, bool EXACT>
+__global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
+{
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int bx = b % A.tiles_x;
+    const int by = (b / A.tiles_x) % A.tiles_y;
+    const int bz = b / (A.tiles_x * A.tiles_y);
+    stage_lut<EXACT>(A.lut, A.nmat, tid, TX * TY);
+    fused_tile<TY, T, EXACT>(A, bx, by, bz);
+}
+
+// Batch tables are read through the constant address space: they do not
+// change while a launch runs, so the block-uniform reads become scalar
+// loads and the grid's pointers stay in SGPRs.
+template <typename T> using Const = const __attribute__((address_space(4))) T;
+template <typename T> __device__ __forceinline__ Const<T> *as_const(const T *p) { return (Const<T> *)p; }
+
+// Step arguments of batch instance d at its step n: the ring slot and H
+// buffer roles follow n exactly as a single grid's do after n steps.
+__device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, StepArgs &A)
+{
+    const int t = n % 3, s1 = (n + 2) % 3, s2 = (n + 1) % 3, ho = n & 1, hn = ho ^ 1;
+    A.hxo = d.H[ho][0]; A.hyo = d.H[ho][1]; A.hzo = d.H[ho][2];
+    A.hxn = d.H[hn][0]; A.hyn = d.H[hn][1]; A.hzn = d.H[hn][2];
+    A.ex0 = d.E[s1][0]; A.ey0 = d.E[s1][1]; A.ez0 = d.E[s1][2];
+    A.ex1 = d.E[s2][0]; A.ey1 = d.E[s2][1]; A.ez1 = d.E[s2][2];
+    A.exT = d.E[t][0]; A.eyT = d.E[t][1]; A.ezT = d.E[t][2];
+    A.dx0 = d.D[s1][0]; A.dy0 = d.D[s1][1]; A.dz0 = d.D[s1][2];
+    A.dx1 = d.D[s2][0]; A.dy1 = d.D[s2][1]; A.dz1 = d.D[s2][2];
+    A.dxT = d.D[t][0]; A.dyT = d.D[t][1]; A.dzT = d.D[t][2];
+    A.ids = d.ids;
+    A.lut = d.lut;
+    A.nmat = d.nmat;
+    const int idx = d.jz_hold3 ? (n / 3) * 3 : n;
+    A.jz = idx < d.jz_len ? d.jz[idx] : 0.0f;
+    A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
+    A.pitch = d.pitch;
+    A.plane = d.plane;
+    A.chx = d.ch[0]; A.chy = d.ch[1]; A.chz = d.ch[2];
+    A.cdx = d.cd[0]; A.cdy = d.cd[1]; A.cdz = d.cd[2];
+    A.kchunk = d.kchunk;
+    A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y; A.tiles_z = d.tiles_z;
+    A.xcd_remap = 0;
+    A.guard = d.guard;
+}
+
+// Persistent fused kernel over every live instance of one storage class:
+// block b runs work items b, b + gridDim.x, ... of the class's tile list
+// (instances in list order, each in z-chunk-major tile order).  Tiles of
+// instances outside their step window are skipped.
+template <int TY, typename T, bool EXACT>
+__global__ __launch_bounds__(TX *TY) void fused_batch(const BatchLaunch L)
+{
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    Const<BatchClass> *C = as_const(L.cls);
+    Const<BatchInst> *I = as_const(L.inst);
+    const int total = C->total;
+    const int step = as_const(L.sh)->step + L.off;
+    int q = 0, staged = -1;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        while (C->start[q + 1] <= t) ++q;
+        const int id = C->idx[q];
+        Const<BatchInst> &d = I[id];
+        const int n = step - d.start;
+        if (n < 0 || n >= d.nsteps) continue;
+        const int local = t - C->start[q];
+        const int layer = d.tiles_x * d.tiles_y;
+        const int bz = local / layer, r = local - bz * layer;
+        const int by = r / d.tiles_x, bx = r - by * d.tiles_x;
+        StepArgs A;
+        batch_step_args(d, n, A);
+        __syncthreads();  // LDS of the previous tile (and its table) no longer read
+        if (id != staged) {
+            stage_lut<EXACT>(A.lut, A.nmat, tid, TX * TY);
+            staged = id;
+        }
+        fused_tile<TY, T, EXACT>(A, bx, by, bz);
+    }
+}
+
 __device__ inline float mur1(float dt, float h, float mu0, float eps0, float rpf, float rpn, float tn, float sn,
                              float sf)
 {
@@ -392,20 +490,19 @@ __device__ inline float mur1(float dt, float h, float mu0, float eps0, float rpf
             (dt - h * sqrtf(mu0 * eps0 * rpf)) * sf);
 }
 
-constexpr int kMurBlock = 256;
-
-template <typename T>
-__global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
+// Face cell t (in segment order) of the 12 (component, face) Mur updates;
+// seg holds the 13 segment prefix sums (M.seg_start is not read).
+template <typename T, typename Seg>
+__device__ __forceinline__ void mur_cell(const MurArgs &M, Seg seg, int t)
 {
     using X = Elem<T>;
     T *exT = (T *)M.exT, *eyT = (T *)M.eyT, *ezT = (T *)M.ezT;
     const T *exS = (const T *)M.exS, *eyS = (const T *)M.eyS, *ezS = (const T *)M.ezS;
-    const int t = blockIdx.x * kMurBlock + threadIdx.x;
-    if (t >= M.seg_start[12]) return;
+    if (t >= seg[12]) return;
     int s = 0;
 #pragma unroll
-    for (int q = 1; q < 12; q++) s += (t >= M.seg_start[q]) ? 1 : 0;
-    const int q = t - M.seg_start[s];
+    for (int q = 1; q < 12; q++) s += (t >= seg[q]) ? 1 : 0;
+    const int q = t - seg[s];
     const int nx = M.nx, ny = M.ny, nz = M.nz;
     const long long sy = M.pitch, sz = M.plane;
     const float dt = M.dt, m0 = M.mu0, e0 = M.eps0;
@@ -483,6 +580,73 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
         break;
     }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
+{
+    mur_cell<T>(M, M.seg_start, blockIdx.x * kMurBlock + threadIdx.x);
+}
+
+// Mur faces of every live instance of a batch: block b runs work items b,
+// b + gridDim.x, ... of the shared Mur list (kMurBlock face cells each).
+__global__ __launch_bounds__(kMurBlock) void mur_batch(const BatchLaunch L)
+{
+    Const<BatchShared> *S = as_const(L.sh);
+    const int total = S->mur_total;
+    const int step = S->step + L.off;
+    int q = 0;
+    for (int blk = blockIdx.x; blk < total; blk += gridDim.x) {
+        while (S->mur_start[q + 1] <= blk) ++q;
+        Const<BatchInst> &d = as_const(L.inst)[S->mur_idx[q]];
+        const int n = step - d.start;
+        if (n < 0 || n >= d.nsteps) continue;
+        const int tt = n % 3, s1 = (n + 2) % 3;
+        MurArgs M;
+        M.exT = d.E[tt][0]; M.eyT = d.E[tt][1]; M.ezT = d.E[tt][2];
+        M.exS = d.E[s1][0]; M.eyS = d.E[s1][1]; M.ezS = d.E[s1][2];
+        M.rpx0 = d.rp[0]; M.rpy0 = d.rp[1]; M.rpz0 = d.rp[2];
+        M.rpxEnd = d.rp[3]; M.rpyEnd = d.rp[4]; M.rpzEnd = d.rp[5];
+        M.nx = d.nx; M.ny = d.ny; M.nz = d.nz; M.pitch = d.pitch; M.plane = d.plane;
+        M.dt = d.dt; M.hx = d.hx; M.hy = d.hy; M.hz = d.hz; M.mu0 = d.mu0; M.eps0 = d.eps0;
+        Const<int> *seg = d.mur_seg;
+        const int t = (blk - S->mur_start[q]) * kMurBlock + (int)threadIdx.x;
+        switch (d.storage) {
+        case kStoreF16: mur_cell<__half>(M, seg, t); break;
+        case kStoreBF16: mur_cell<__hip_bfloat16>(M, seg, t); break;
+        default: mur_cell<float>(M, seg, t); break;
+        }
+    }
+}
+
+// Probe records of a batch: record r = (instance, probe) writes row n of the
+// instance's series after its step n.
+__global__ __launch_bounds__(256) void probe_batch(const BatchLaunch L)
+{
+    const BatchShared *S = L.sh;
+    const int step = S->step + L.off;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < S->nrec; r += gridDim.x * blockDim.x) {
+        const int id = S->rec[r][0], p = S->rec[r][1];
+        const BatchInst &d = L.inst[id];
+        const int n = step - d.start;
+        if (n < 0 || n >= d.nsteps || n >= d.probe_rows) continue;
+        const int *q = d.probes + 4 * p;
+        const int f = q[0];
+        const long long o = (long long)q[3] * d.plane + (long long)q[2] * d.pitch + q[1];
+        const void *base = f < 3 ? d.D[n % 3][f] : (f < 6 ? d.H[(n & 1) ^ 1][f - 3] : d.E[n % 3][f - 6]);
+        float v;
+        switch (d.storage) {
+        case kStoreF16: v = Elem<__half>::ld((const __half *)base, (uint32_t)(o * 2)); break;
+        case kStoreBF16: v = Elem<__hip_bfloat16>::ld((const __hip_bfloat16 *)base, (uint32_t)(o * 2)); break;
+        default: v = Elem<float>::ld((const float *)base, (uint32_t)(o * 4)); break;
+        }
+        d.probe_out[(long long)n * d.nprobe + p] = (double)v * d.factor[f];
+    }
+}
+
+__global__ void batch_advance(BatchShared *sh, int steps)
+{
+    if (threadIdx.x == 0) sh->step += steps;
 }
 
 template <typename T>
@@ -565,4 +729,89 @@ hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t
     case kStoreBF16: return launch_fused_t<__hip_bfloat16, false>(a, ty, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+
+template <int TY, typename T, bool EXACT>
+hipError_t launch_fused_batch_t(const BatchLaunch &l, int grid, hipStream_t s)
+{
+    fused_batch<TY, T, EXACT><<<grid, dim3(TX, TY), batch_fused_lds(), s>>>(l);
+    return hipGetLastError();
+}
+
+template <typename T, bool EXACT>
+hipError_t launch_fused_batch_s(const BatchLaunch &l, int ty, int grid, hipStream_t s)
+{
+    switch (ty) {
+    case 2: return launch_fused_batch_t<2, T, EXACT>(l, grid, s);
+    case 4: return launch_fused_batch_t<4, T, EXACT>(l, grid, s);
+    case 8: return launch_fused_batch_t<8, T, EXACT>(l, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int TY, typename T, bool EXACT>
+int occupancy_t()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fused_batch<TY, T, EXACT>, TX * TY, batch_fused_lds()) !=
+        hipSuccess)
+        return 0;
+    return nb;
+}
+
+template <typename T, bool EXACT>
+int occupancy_s(int ty)
+{
+    switch (ty) {
+    case 2: return occupancy_t<2, T, EXACT>();
+    case 4: return occupancy_t<4, T, EXACT>();
+    case 8: return occupancy_t<8, T, EXACT>();
+    default: return 0;
+    }
+}
+
+}  // namespace
+
+size_t batch_fused_lds() { return (size_t)kMaxMaterials * (3 * sizeof(double) + 3 * sizeof(float)); }
+
+hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int storage, int grid, hipStream_t s)
+{
+    switch (storage) {
+    case kStoreF32: return launch_fused_batch_s<float, true>(l, ty, grid, s);
+    case kStoreF32Fast: return launch_fused_batch_s<float, false>(l, ty, grid, s);
+    case kStoreF16: return launch_fused_batch_s<__half, false>(l, ty, grid, s);
+    case kStoreBF16: return launch_fused_batch_s<__hip_bfloat16, false>(l, ty, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int batch_fused_occupancy(int ty, int storage)
+{
+    switch (storage) {
+    case kStoreF32: return occupancy_s<float, true>(ty);
+    case kStoreF32Fast: return occupancy_s<float, false>(ty);
+    case kStoreF16: return occupancy_s<__half, false>(ty);
+    case kStoreBF16: return occupancy_s<__hip_bfloat16, false>(ty);
+    default: return 0;
+    }
+}
+
+hipError_t launch_mur_batch(const BatchLaunch &l, int grid, hipStream_t s)
+{
+    mur_batch<<<grid, kMurBlock, 0, s>>>(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_batch(const BatchLaunch &l, int grid, hipStream_t s)
+{
+    probe_batch<<<grid, 256, 0, s>>>(l);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s)
+{
+    batch_advance<<<1, 64, 0, s>>>(sh, steps);
+    return hipGetLastError();
 }

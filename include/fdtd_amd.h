@@ -37,6 +37,7 @@ extern "C" {
 #define FDTD_ERANGE   (-2)  /* index or size out of range */
 #define FDTD_ESTATE   (-3)  /* call out of order */
 #define FDTD_EIO      (-4)  /* file could not be read */
+#define FDTD_ENOMEM   (-5)  /* does not fit the device memory budget or the slot table */
 
 /* Field ids, in the column order of the reference's probe writer
  * (fdtd.cu:1223). */
@@ -132,6 +133,143 @@ int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, 
  * fdtd_ctx_kernel_time waits for them and returns the summed duration. */
 int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
+
+/* ---- batches of independent grids --------------------------------------
+ *
+ * A batch holds many grids of different sizes, materials, sources, storage
+ * forms and step counts in one device arena.  Each step of the batch is one
+ * persistent fused kernel per storage form present, one Mur launch and one
+ * probe launch over every running grid, replayed from captured HIP graphs;
+ * grids join (fdtd_batch_add) and leave (fdtd_batch_remove) between calls.
+ * A grid's results are those of a single fdtd_ctx with the same inputs, bit
+ * for bit (the per-cell arithmetic does not depend on the batch). */
+
+typedef struct fdtd_batch fdtd_batch;
+
+#define FDTD_BATCH_MAX_INSTANCES 512
+#define FDTD_BATCH_MAX_PROBES 64
+
+typedef struct
+{
+    const unsigned char *ids;   /* nx*ny*nz material ids (x fastest), or NULL: all material 0 */
+    int nmat;                   /* rows of table, 1..128 */
+    const float *table;         /* nmat x {sigma, eps_s, eps_inf, tau_d} */
+    const float *rp[6];         /* Mur planes rpx0 rpy0 rpz0 rpxEnd rpyEnd rpzEnd; NULL: all 1 */
+    int nsrc;
+    const int *src_xyz;         /* nsrc x (x, y, z), 0-based, as fdtd_ctx_set_sources */
+    int jz_len;
+    const float *jz;            /* source waveform */
+    int steps;                  /* time steps to run, >= 1 */
+    int nprobe;                 /* 0..FDTD_BATCH_MAX_PROBES */
+    const int *probes;          /* nprobe x (field, i, j, k); one row per step */
+} FdtdInstance;
+
+typedef struct
+{
+    int instances;              /* added and not removed */
+    int running;                /* of them, not yet at their step count */
+    long long step;             /* batch steps run */
+    size_t arena_bytes;         /* device arena (the memory budget) */
+    size_t used_bytes;
+    size_t largest_free;        /* largest region an instance can still get */
+    int tile_y, kchunk;         /* fused launch shape */
+    int graph_steps;            /* steps per captured graph */
+    int grid[4];                /* persistent blocks per fused launch, per storage form */
+    int tiles[4];               /* fused tiles per step of the running grids, per storage form */
+    long long cells[4];         /* cells of the running grids, per storage form */
+} FdtdBatchInfo;
+
+/* hbm_bytes: size of the device arena that holds every grid's state. */
+int fdtd_batch_create(size_t hbm_bytes, fdtd_batch **out);
+int fdtd_batch_destroy(fdtd_batch *b);
+/* Arena bytes fdtd_batch_add would take for this grid. */
+int fdtd_batch_instance_bytes(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst, size_t *bytes);
+/* Uploads a grid (zero initial fields, as copyDataToDevice does) and gives
+ * it an id; it takes its first step in the next fdtd_batch_run.  Of ext,
+ * debye_mode, jz_index_mode, jz_limit, storage and the scales apply; the
+ * launch shape is the batch's.  FDTD_ENOMEM when the arena or the slot
+ * table is full. */
+int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst, int *id);
+/* Waits for the batch's work and frees the grid's arena region. */
+int fdtd_batch_remove(fdtd_batch *b, int id);
+/* Runs `steps` batch steps on `stream` (enqueue only); a grid stops at its
+ * own step count. */
+int fdtd_batch_run(fdtd_batch *b, int steps, hipStream_t stream);
+int fdtd_batch_status(fdtd_batch *b, int id, int *steps_done, int *steps_total);
+/* Waits for the device; copies up to max_rows probe rows; returns rows. */
+int fdtd_batch_probe_read(fdtd_batch *b, int id, double *host, int max_rows);
+/* Waits for the device; level as fdtd_ctx_get_field, after the grid's
+ * steps so far. */
+int fdtd_batch_get_field(fdtd_batch *b, int id, int field, int level, float *host);
+/* tile_y in {2,4,8}, kchunk >= 1, graph_steps >= 1 (0 keeps a value). */
+int fdtd_batch_set_shape(fdtd_batch *b, int tile_y, int kchunk, int graph_steps);
+/* Times every (tile_y, kchunk) candidate on the batch's own grids for
+ * trial_steps steps each, keeps the fastest and restores the grids to
+ * their zero initial state.  Every grid must be unstepped. */
+int fdtd_batch_autotune(fdtd_batch *b, int trial_steps, hipStream_t stream, int *tile_y, int *kchunk);
+int fdtd_batch_info(fdtd_batch *b, FdtdBatchInfo *info);
+
+/* ---- serving ------------------------------------------------------------
+ *
+ * A server owns a batch and a worker thread.  Requests (a grid, its inputs
+ * and probes) queue on submit; between graph replays the worker admits
+ * queued requests in arrival order while the slot limit and the memory
+ * budget allow (a request that does not fit lets later ones that fit pass),
+ * steps every admitted grid together, and retires finished ones: their
+ * probe series are copied out and their memory returns to the budget. */
+
+typedef struct fdtd_serve fdtd_serve;
+
+typedef struct
+{
+    size_t hbm_bytes;           /* memory budget (the batch arena) */
+    int max_slots;              /* grids on the device at once, <= FDTD_BATCH_MAX_INSTANCES */
+    int segment_steps;          /* steps per replay between admissions (0 = 8) */
+    int tile_y, kchunk;         /* fused launch shape (0 = defaults) */
+} FdtdServeConfig;
+
+#define FDTD_REQ_QUEUED  0
+#define FDTD_REQ_RUNNING 1
+#define FDTD_REQ_DONE    2
+#define FDTD_REQ_FAILED  3
+
+typedef struct
+{
+    int state;                  /* FDTD_REQ_* */
+    int error;                  /* FDTD_E* or hipError_t when FAILED */
+    int rows, nprobe;           /* probe series shape */
+    double t_submit, t_admit, t_done;   /* seconds since fdtd_serve_create */
+    long long cells;            /* nx*ny*nz */
+    int steps;
+} FdtdRequestInfo;
+
+typedef struct
+{
+    int submitted, completed, failed, queued, running;
+    double t_first_submit, t_last_done;   /* seconds since fdtd_serve_create */
+    double solves_per_s;        /* completed / (t_last_done - t_first_submit) */
+    double p50_latency_s, p99_latency_s, mean_latency_s;   /* t_done - t_submit of completed */
+    double cell_updates;        /* sum of cells x steps of completed */
+    long long segments;         /* graph replays */
+    size_t peak_bytes;          /* largest arena use */
+    int peak_running;
+} FdtdServeStats;
+
+int fdtd_serve_create(const FdtdServeConfig *cfg, fdtd_serve **out);
+/* Stops the worker after the admitted grids finish; queued requests fail. */
+int fdtd_serve_destroy(fdtd_serve *s);
+/* Copies the inputs; thread-safe; returns at once. */
+int fdtd_serve_submit(fdtd_serve *s, const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst,
+                      long long *request);
+/* 0 when done or failed, 1 on timeout (timeout_s < 0: wait without limit). */
+int fdtd_serve_wait(fdtd_serve *s, long long request, double timeout_s);
+/* Copies up to max_rows probe rows of a finished request; returns rows. */
+int fdtd_serve_result(fdtd_serve *s, long long request, double *probes, int max_rows, FdtdRequestInfo *info);
+/* Frees a finished request's host results. */
+int fdtd_serve_release(fdtd_serve *s, long long request);
+/* Waits until nothing is queued or running (1 on timeout). */
+int fdtd_serve_drain(fdtd_serve *s, double timeout_s);
+int fdtd_serve_stats(fdtd_serve *s, FdtdServeStats *st);
 
 #ifdef __cplusplus
 }

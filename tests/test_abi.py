@@ -50,3 +50,36 @@ def test_no_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(F.FdtdError):
         F.Context(F.make_params((8, 8, 8), 1e-12, 1e-3, mu0=1.2566371e-06, eps0=8.841941e-12))
+
+
+EXT_STRUCTS = {
+    "FdtdExt": [],
+    "FdtdInstance": ["rp", "nsrc", "jz", "steps", "probes"],
+    "FdtdBatchInfo": ["step", "tile_y", "grid", "cells"],
+    "FdtdServeConfig": ["max_slots", "kchunk"],
+    "FdtdRequestInfo": ["t_submit", "cells", "steps"],
+    "FdtdServeStats": ["t_first_submit", "segments", "peak_bytes", "peak_running"],
+}
+
+
+def test_extension_struct_layout(tmp_path):
+    """The ctypes mirrors of fdtd_amd.h's structs match what a C compiler
+    lays out (sizes and field offsets)."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fdtd_amd.h"', "int main(void) {"]
+    for name, fields in EXT_STRUCTS.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for f in fields:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", "-I/opt/rocm/include", str(src),
+                    "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+               .stdout.splitlines())
+    for name, fields in EXT_STRUCTS.items():
+        S = getattr(F, name)
+        assert int(got[name]) == C.sizeof(S), name
+        for f in fields:
+            assert int(got[f"{name}.{f}"]) == getattr(S, f).offset, f"{name}.{f}"

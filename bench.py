@@ -12,12 +12,25 @@ kernel's roofline (97 algorithmic bytes per cell-update over its mean
 launch duration, timed with HIP events on its stream) and the CPU baseline
 (the oracle's FP32 transcription, OpenMP on this host's cores, on a bounded
 sample of the same workload).
+
+Two more workloads measure the systems patterns:
+  --workload batched  64 uneven grids (batched_set) stepped together, one
+                      persistent fused launch per step from captured graphs,
+                      shape picked by the autotuner; whole-job Gcell-updates/s
+                      beside running the same grids one after another.
+  --workload served   an open-loop Poisson stream of solves (served_stream)
+                      through the continuous-batching server: solves/s and
+                      p50/p99 latency beside the sequential drop-in path (the
+                      five reference entry points, one solve at a time, FIFO)
+                      on the same stream.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -31,7 +44,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="plain_256", choices=["plain_256", "large_phantom", "plain_small"])
+    ap.add_argument("--workload", default="plain_256",
+                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served"])
+    ap.add_argument("--requests", type=int, default=120, help="served: requests of the stream")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="served: device memory budget")
+    ap.add_argument("--slots", type=int, default=64, help="served: grids on the device at once")
+    ap.add_argument("--segment", type=int, default=8, help="batched/served: steps per graph replay")
+    ap.add_argument("--no-sequential", action="store_true", help="batched/served: skip the sequential baselines")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="field storage (f32 reproduces the reference's arithmetic bit for bit)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU baseline sample")
@@ -86,6 +105,11 @@ def main():
         hip.sync()
         src.free()
         dst.free()
+
+    if a.workload == "batched":
+        return bench_batched(a)
+    if a.workload == "served":
+        return bench_served(a)
 
     total = a.warmup + a.steps
     w = build_workload(a.workload, total)
@@ -149,6 +173,174 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps)
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def _instance(F, w, storage="f32"):
+    ext = F.make_ext(storage={"f32": F.STORE_F32, "f16": F.STORE_F16}[storage])
+    return F.Instance(w.params, ext, w.ids, w.table, w.sources, w.jz, w.steps, w.probes)
+
+
+def _pct(v, q):
+    v = sorted(v)
+    k = min(max(int(-(-q * len(v) // 1)), 1), len(v))
+    return v[k - 1]
+
+
+def _sequential_fused(F, hip, insts):
+    """Each grid alone in a fused context, one after another (set-up, run,
+    probe read-back and free per grid)."""
+    times = []
+    for inst in insts:
+        t0 = time.perf_counter()
+        e = inst.ext
+        ctx = F.Context(inst.params, debye=e.debye_mode, jz_mode=e.jz_index_mode, storage=e.storage)
+        ctx.set_materials(inst.ids, inst.table)
+        ctx.set_sources(inst.src, inst.jz)
+        ctx.set_probes(inst.probes, inst.steps)
+        ctx.run(inst.steps)
+        ctx.probes()
+        ctx.close()
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def _sequential_dropin(F, insts):
+    """Each grid through the five drop-in entry points, one after another
+    (fp32, the reference's 512-value jz table)."""
+    times = []
+    for inst in insts:
+        ext = F.make_ext(jz_limit=512)
+        x = F.Instance(inst.params, ext, inst.ids, inst.table, inst.src, inst.jz, inst.steps, inst.probes)
+        t0 = time.perf_counter()
+        F.dropin_solve(x)
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def bench_batched(a):
+    import fdtd_amd as F
+    from fdtd_amd import hip
+    from fdtd_amd import workloads as W
+
+    ws = W.batched_set()
+    insts = [_instance(F, w) for w in ws]
+    work = sum(w.cells * w.steps for w in ws)
+    cells = sum(w.cells for w in ws)
+    arena = int(sum(x.nbytes() for x in insts) * 1.02) + (64 << 20)
+    b = F.Batch(arena)
+    b.set_shape(graph_steps=a.segment)
+    t0 = time.perf_counter()
+    ids = [b.add(x) for x in insts]
+    t_upload = time.perf_counter() - t0
+    stream = hip.Stream()
+    ty, kc = b.autotune(trial_steps=4, stream=stream.handle)
+    info = b.info()
+    steps = max(w.steps for w in ws)
+    e0, e1 = hip.Event(), hip.Event()
+    hip.sync()
+    e0.record(stream)
+    b.run(steps, stream.handle)
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1)
+    done = [b.status(i)[0] == x.steps for i, x in zip(ids, insts)]
+    series = [b.probes(i) for i in ids]
+    ok = all(done) and all(np.isfinite(s).all() for s in series)
+    b.close()
+    value = work / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s (64 uneven grids batched: one persistent fused launch per step)",
+        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 Debye sums)", "data": "synthetic (seeded phantoms)",
+        "config": {"workload": "batched_set", "grids": len(ws), "cells": cells, "cell_updates": work,
+                   "steps_range": [min(w.steps for w in ws), steps], "tile_y": ty, "kchunk": kc,
+                   "graph_steps": a.segment, "persistent_grid": info["grid"][0], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(97 * work / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(97 * work / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "whole batch step (fused_batch + mur_batch + probe_batch)",
+                     "bytes_per_cell": 97},
+        "upload_s": round(t_upload, 3), "all_finished": ok,
+    }
+    if not a.no_sequential:
+        tf = _sequential_fused(F, hip, insts)
+        td = _sequential_dropin(F, insts)
+        out["sequential_fused"] = {"seconds": round(sum(tf), 3), "gcell_per_s": round(work / sum(tf) / 1e9, 3)}
+        out["sequential_dropin"] = {"seconds": round(sum(td), 3), "gcell_per_s": round(work / sum(td) / 1e9, 3)}
+        out["speedup_vs_sequential_dropin"] = round(sum(td) / (ms * 1e-3), 3)
+        out["speedup_vs_sequential_fused"] = round(sum(tf) / (ms * 1e-3), 3)
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(a):
+    import threading
+
+    import fdtd_amd as F
+    from fdtd_amd import hip
+    from fdtd_amd import workloads as W
+
+    stream = W.served_stream(a.requests)
+    reqs = [(t, _instance(F, w, st), st) for t, w, st in stream]
+    work = sum(x.params.nx * x.params.ny * x.params.nz * x.steps for _, x, _ in reqs)
+    srv = F.Server(int(a.budget_gb * (1 << 30)), max_slots=a.slots, segment_steps=a.segment)
+    ids = [None] * len(reqs)
+
+    def client():
+        t0 = time.perf_counter()
+        for q, (t, x, _) in enumerate(reqs):
+            d = t - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids[q] = srv.submit(x)
+            if (q + 1) % 40 == 0:
+                print(f"[served] submitted {q + 1}/{len(reqs)}", file=sys.stderr, flush=True)
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    while not srv.drain(30.0):
+        st = srv.stats()
+        print(f"[served] completed {st['completed']}/{len(reqs)}", file=sys.stderr, flush=True)
+    st = srv.stats()
+    finite = True
+    for r in ids:
+        s, info = srv.result(r)
+        finite &= bool(np.isfinite(s).all()) and info["state"] == F.REQ_DONE
+    srv.close()
+    out = {
+        "metric": "served solves/s (continuous batching, open-loop Poisson 40 req/s) with p50/p99 latency",
+        "value": round(st["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": a.requests, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 Debye sums) or f16 storage per request", "data": "synthetic (seeded phantoms)",
+        "config": {"workload": "served_stream", "requests": len(reqs), "rate_per_s": 40.0,
+                   "f16_requests": sum(st_ == "f16" for *_, st_ in reqs), "cell_updates": work,
+                   "budget_gb": a.budget_gb, "slots": a.slots, "segment_steps": a.segment,
+                   "parallelism": "single GPU"},
+        "p50_latency_s": round(st["p50_latency_s"], 3), "p99_latency_s": round(st["p99_latency_s"], 3),
+        "mean_latency_s": round(st["mean_latency_s"], 3),
+        "gcell_per_s": round(st["cell_updates"] / max(st["t_last_done"] - st["t_first_submit"], 1e-9) / 1e9, 3),
+        "peak_running": st["peak_running"], "peak_gb": round(st["peak_bytes"] / (1 << 30), 2),
+        "completed": st["completed"], "failed": st["failed"], "results_finite": finite,
+    }
+    if not a.no_sequential:
+        arrivals = [t for t, _, _ in reqs]
+
+        def fifo(service):
+            fin, lat, t = [], [], 0.0
+            for arr, s in zip(arrivals, service):
+                t = max(t, arr) + s
+                fin.append(t)
+                lat.append(t - arr)
+            return {"seconds": round(fin[-1] - arrivals[0], 3), "solves_per_s": round(len(fin) / (fin[-1] - arrivals[0]), 3),
+                    "p50_latency_s": round(_pct(lat, 0.5), 3), "p99_latency_s": round(_pct(lat, 0.99), 3)}
+
+        td = _sequential_dropin(F, [x for _, x, _ in reqs])
+        out["sequential_dropin"] = fifo(td)
+        out["speedup_vs_sequential_dropin"] = round(out["value"] / out["sequential_dropin"]["solves_per_s"], 3)
+        tf = _sequential_fused(F, hip, [x for _, x, _ in reqs])
+        out["sequential_fused"] = fifo(tf)
+        out["speedup_vs_sequential_fused"] = round(out["value"] / out["sequential_fused"]["solves_per_s"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -46,10 +46,11 @@ EXPORTS = {
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
-                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales",
+                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
+                   "fdtd_dropin_solve",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
@@ -163,6 +164,7 @@ def lib():
         L.fdtd_ctx_set_timing.argtypes = [vp, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
+        L.fdtd_ctx_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         PP, PE, PI = C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(FdtdInstance)
         L.fdtd_batch_create.argtypes = [C.c_size_t, C.POINTER(vp)]
         L.fdtd_batch_destroy.argtypes = [vp]
@@ -176,6 +178,7 @@ def lib():
         L.fdtd_batch_set_shape.argtypes = [vp, i, i, i]
         L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
+        L.fdtd_dropin_solve.argtypes = [PP, PE, PI, vp, vp]
         L.fdtd_serve_create.argtypes = [C.POINTER(FdtdServeConfig), C.POINTER(vp)]
         L.fdtd_serve_destroy.argtypes = [vp]
         L.fdtd_serve_submit.argtypes = [vp, PP, PE, PI, C.POINTER(C.c_longlong)]
@@ -341,6 +344,12 @@ class Context:
         check(lib().fdtd_ctx_scales(self._h, *[C.byref(x) for x in v]), "scales")
         return tuple(x.value for x in v)
 
+    def layout(self):
+        """(pitch, plane, element bytes) of the device fields."""
+        p, pl, e = C.c_int(), C.c_longlong(), C.c_int()
+        check(lib().fdtd_ctx_layout(self._h, C.byref(p), C.byref(pl), C.byref(e)), "layout")
+        return p.value, pl.value, e.value
+
     def launch_shape(self):
         v = [C.c_int() for _ in range(4)]
         check(lib().fdtd_ctx_launch_shape(self._h, *[C.byref(x) for x in v]), "launch_shape")
@@ -396,6 +405,15 @@ class Instance:
         check(lib().fdtd_batch_instance_bytes(C.byref(self.params), C.byref(self.ext), C.byref(self.c),
                                               C.byref(b)), "instance_bytes")
         return b.value
+
+
+def dropin_solve(inst, stream=None):
+    """Probe series [steps, nprobe] of one grid solved through the five
+    drop-in entry points (fdtd_dropin_solve)."""
+    out = np.zeros((inst.steps, max(inst.nprobe, 1)), np.float64)
+    check(lib().fdtd_dropin_solve(C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c), _ptr(out), stream),
+          "fdtd_dropin_solve")
+    return out[:, :inst.nprobe]
 
 
 class Batch:

@@ -33,7 +33,9 @@ struct DevTables {
 
 // Byte layout of one grid's arena region.
 struct Layout {
-    size_t N, esz, skew;
+    size_t N, Np, esz, skew;       // cells, elements per field array, bytes per element, array stagger
+    int pitch;
+    long long plane;
     long long guard;
     size_t blob, blob_bytes, ids, lut, jz, rp[6], probes, out, out_bytes, total;
     int jz_n;
@@ -44,17 +46,20 @@ Layout layout_of(int nx, int ny, int nz, int storage, int nmat, int jz_n, int np
     Layout L;
     L.N = (size_t)nx * ny * nz;
     L.esz = fdtd_storage_bytes(storage);
+    L.pitch = fused_pitch(nx, L.esz);
+    L.plane = (long long)L.pitch * ny;
+    L.Np = (size_t)L.plane * nz;
     // Channel stagger between the 24 field arrays of large grids (as in a
     // single context); small grids are not worth the memory.
-    L.skew = L.N >= (size_t)1 << 20 ? 65536 : 0;
-    L.guard = fused_guard(nx, (long long)nx * ny);
+    L.skew = L.Np >= (size_t)1 << 20 ? 65536 : 0;
+    L.guard = fused_guard(L.pitch, L.plane);
     L.jz_n = jz_n;
     size_t o = 0;
     L.blob = o;
-    L.blob_bytes = (24 * (L.N + L.skew) + 2 * (size_t)L.guard) * L.esz;
+    L.blob_bytes = (24 * (L.Np + L.skew) + 2 * (size_t)L.guard) * L.esz;
     o = round_up(o + L.blob_bytes, 256);
     L.ids = o;
-    o = round_up(o + L.N + 2 * (size_t)L.guard, 256);
+    o = round_up(o + L.Np + 2 * (size_t)L.guard, 256);
     L.lut = o;
     o = round_up(o + (size_t)nmat * sizeof(MatCoef), 256);
     L.jz = o;
@@ -83,10 +88,11 @@ int check_instance(const FdtdParams *p, const FdtdExt &e, const FdtdInstance *in
 {
     if (!p || !in) return fdtd_fail(FDTD_EINVAL, "null argument");
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fdtd_fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
-    const long long N = (long long)p->nx * p->ny * p->nz;
-    if (N + 2 * fused_guard(p->nx, (long long)p->nx * p->ny) >= (1LL << 30))
-        return fdtd_fail(FDTD_ERANGE, "grid of %lld cells: too large", N);
     if (int rc = fdtd_check_ext(e)) return rc;
+    const int pitch = fused_pitch(p->nx, fdtd_storage_bytes(e.storage));
+    const long long plane = (long long)pitch * p->ny;
+    if (plane * p->nz + 2 * fused_guard(pitch, plane) >= (1LL << 30))
+        return fdtd_fail(FDTD_ERANGE, "grid of %lld cells: too large", (long long)p->nx * p->ny * p->nz);
     if (in->nmat < 1 || in->nmat > kMaxMaterials || !in->table) return fdtd_fail(FDTD_ERANGE, "nmat %d not in [1,%d]", in->nmat, kMaxMaterials);
     if (in->steps < 1) return fdtd_fail(FDTD_ERANGE, "steps %d", in->steps);
     if (in->nsrc < 0 || (in->nsrc > 0 && !in->src_xyz) || in->jz_len < 0 || (in->jz_len > 0 && !in->jz))
@@ -431,11 +437,11 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
 
     char *base = b->arena_base + off;
     // ids with source bits, coefficient table, stored waveform, Mur planes, probes
-    std::vector<unsigned char> ids(N, 0);
-    if (in->ids) memcpy(ids.data(), in->ids, N);
+    std::vector<unsigned char> ids(L.Np, 0);
+    if (in->ids) fdtd_pitch_bytes(in->ids, nx, (size_t)ny * nz, L.pitch, ids.data());
     for (int s = 0; s < in->nsrc; s++) {
         const int *q = in->src_xyz + 3 * s;
-        ids[((size_t)q[2] * ny + q[1]) * nx + q[0]] |= kSourceBit;
+        ids[(size_t)q[2] * L.plane + (size_t)q[1] * L.pitch + q[0]] |= kSourceBit;
     }
     std::vector<MatCoef> lut(in->nmat);
     fdtd_build_lut(in->table, in->nmat, p->dt, p->eps0, e.debye_mode, e.storage, I.sE, I.sD, lut.data());
@@ -444,9 +450,9 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     const size_t rps[6] = {(size_t)2 * ny * nz, (size_t)nx * 2 * nz, (size_t)nx * ny * 2,
                            (size_t)2 * ny * nz, (size_t)nx * 2 * nz, (size_t)nx * ny * 2};
     std::vector<float> ones;
-    hipError_t he = hipMemsetAsync(base, 0, L.ids + N + 2 * (size_t)L.guard, b->up);
+    hipError_t he = hipMemsetAsync(base, 0, L.ids + L.Np + 2 * (size_t)L.guard, b->up);
     if (he == hipSuccess) he = hipMemsetAsync(base + L.out, 0, L.out_bytes, b->up);
-    if (he == hipSuccess) he = hipMemcpyAsync(base + L.ids + L.guard, ids.data(), N, hipMemcpyHostToDevice, b->up);
+    if (he == hipSuccess) he = hipMemcpyAsync(base + L.ids + L.guard, ids.data(), L.Np, hipMemcpyHostToDevice, b->up);
     if (he == hipSuccess) he = hipMemcpyAsync(base + L.lut, lut.data(), sizeof(MatCoef) * in->nmat, hipMemcpyHostToDevice, b->up);
     if (he == hipSuccess && jz_n > 0) he = hipMemcpyAsync(base + L.jz, jz.data(), sizeof(float) * jz_n, hipMemcpyHostToDevice, b->up);
     for (int r = 0; r < 6 && he == hipSuccess; r++) {
@@ -469,7 +475,7 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     BatchInst &d = I.d;
     memset(&d, 0, sizeof d);
     char *blob = base + L.blob + (size_t)L.guard * L.esz;
-    const size_t stride = (L.N + L.skew) * L.esz;
+    const size_t stride = (L.Np + L.skew) * L.esz;
     for (int s = 0; s < 3; s++)
         for (int q = 0; q < 3; q++) d.E[s][q] = blob + (size_t)(s * 3 + q) * stride;
     for (int s = 0; s < 3; s++)
@@ -483,14 +489,14 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     d.probes = (const int *)(base + L.probes);
     d.probe_out = (double *)(base + L.out);
     for (int f = 0; f < 9; f++) d.factor[f] = fdtd_field_factor(e.storage, p->mu0, p->eps0, I.sE, I.sD, I.sH, f);
-    d.plane = (long long)nx * ny;
+    d.plane = L.plane;
     d.nmat = in->nmat;
     d.jz_len = jz_n;
     d.jz_hold3 = e.jz_index_mode == FDTD_JZ_HOLD3;
     d.start = (int)b->step;
     d.nsteps = in->steps;
     d.storage = e.storage;
-    d.nx = nx; d.ny = ny; d.nz = nz; d.pitch = nx;
+    d.nx = nx; d.ny = ny; d.nz = nz; d.pitch = L.pitch;
     d.guard = (unsigned)L.guard;
     const GridGeom g{p->dt, p->dx, p->dy, p->dz, p->mu0, p->eps0};
     fdtd_curl_coefs(g, e.storage, I.sE, I.sD, I.sH, d.ch, d.cd);
@@ -596,9 +602,9 @@ extern "C" int fdtd_batch_get_field(fdtd_batch *b, int id, int field, int level,
         const int s = ((n + 2 - level) % 3 + 3) % 3;
         src = field < 3 ? I.d.D[s][field] : I.d.E[s][field - 6];
     }
-    std::vector<char> buf(I.L.N * I.L.esz);
+    std::vector<char> buf(I.L.Np * I.L.esz);
     HIP_TRY(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
-    fdtd_from_storage(I.d.storage, buf.data(), I.L.N, I.d.factor[field], host);
+    fdtd_from_storage_pitched(I.d.storage, buf.data(), I.d.nx, (size_t)I.d.ny * I.d.nz, I.L.pitch, I.d.factor[field], host);
     return 0;
 }
 

@@ -263,3 +263,29 @@ int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes)
     }
     return 0;
 }
+
+void fdtd_to_storage_pitched(int storage, const float *in, int nx, size_t rows, int pitch, double factor, void *out)
+{
+    const size_t esz = fdtd_storage_bytes(storage);
+    char *o = (char *)out;
+    for (size_t r = 0; r < rows; r++) {
+        char *row = o + r * pitch * esz;
+        fdtd_to_storage(storage, in + r * nx, nx, factor, row);
+        memset(row + (size_t)nx * esz, 0, (size_t)(pitch - nx) * esz);
+    }
+}
+
+void fdtd_from_storage_pitched(int storage, const void *in, int nx, size_t rows, int pitch, double factor, float *out)
+{
+    const size_t esz = fdtd_storage_bytes(storage);
+    const char *i = (const char *)in;
+    for (size_t r = 0; r < rows; r++) fdtd_from_storage(storage, i + r * pitch * esz, nx, factor, out + r * nx);
+}
+
+void fdtd_pitch_bytes(const unsigned char *in, int nx, size_t rows, int pitch, unsigned char *out)
+{
+    for (size_t r = 0; r < rows; r++) {
+        memcpy(out + r * pitch, in + r * nx, nx);
+        memset(out + r * pitch + nx, 0, pitch - nx);
+    }
+}

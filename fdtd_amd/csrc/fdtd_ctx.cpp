@@ -29,9 +29,10 @@ constexpr size_t kArraySkew = 65536;
 
 struct fdtd_ctx {
     int nx, ny, nz;
-    size_t N;
-    int pitch;
-    long long plane;
+    size_t N;                            // cells
+    size_t Np;                           // elements of one field array (plane * nz)
+    int pitch;                           // elements between rows (fused_pitch)
+    long long plane;                     // pitch * ny
     float dt, dx, dy, dz, mu0, eps0;
     FdtdExt ext;
     int storage;
@@ -99,7 +100,9 @@ int upload_ids(fdtd_ctx *c)
         const size_t o = ((size_t)c->src[3 * s + 2] * ny + c->src[3 * s + 1]) * nx + c->src[3 * s];
         merged[o] |= kSourceBit;
     }
-    HIP_TRY(hipMemcpy(c->ids, merged.data(), c->N, hipMemcpyHostToDevice));
+    std::vector<unsigned char> img(c->Np);
+    fdtd_pitch_bytes(merged.data(), c->nx, (size_t)c->ny * c->nz, c->pitch, img.data());
+    HIP_TRY(hipMemcpy(c->ids, img.data(), c->Np, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -145,20 +148,23 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     *out = nullptr;
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fdtd_fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
     const long long N = (long long)p->nx * p->ny * p->nz;
-    // The fused kernel addresses each field array with 32-bit byte offsets
-    // that span the array and its guards.
-    if (N + 2 * fused_guard(p->nx, (long long)p->nx * p->ny) >= (1LL << 30))
-        return fdtd_fail(FDTD_ERANGE, "grid of %lld cells: too large for one context", N);
     FdtdExt e;
     fdtd_ext_defaults(&e);
     if (ext) e = *ext;
     if (int rc = fdtd_check_ext(e)) return rc;
+    // The fused kernel addresses each field array with 32-bit byte offsets
+    // that span the array and its guards.
+    const int pitch = fused_pitch(p->nx, fdtd_storage_bytes(e.storage));
+    const long long plane = (long long)pitch * p->ny;
+    if (plane * p->nz + 2 * fused_guard(pitch, plane) >= (1LL << 30))
+        return fdtd_fail(FDTD_ERANGE, "grid of %lld cells: too large for one context", N);
 
     fdtd_ctx *c = new fdtd_ctx();
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz;
     c->N = (size_t)N;
-    c->pitch = p->nx;
-    c->plane = (long long)p->nx * p->ny;
+    c->pitch = pitch;
+    c->plane = plane;
+    c->Np = (size_t)plane * p->nz;
     c->dt = p->dt; c->dx = p->dx; c->dy = p->dy; c->dz = p->dz; c->mu0 = p->mu0; c->eps0 = p->eps0;
     c->ext = e;
     c->storage = e.storage;
@@ -173,11 +179,11 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H
     c->guard = fused_guard(c->pitch, c->plane);
-    const size_t blob_elems = nf * (c->N + kArraySkew) + 2 * (size_t)c->guard;
+    const size_t blob_elems = nf * (c->Np + kArraySkew) + 2 * (size_t)c->guard;
     hipError_t he = hipMalloc(&c->blob, blob_elems * c->esz);
     if (he != hipSuccess) { delete c; return fdtd_hip_fail(he, "hipMalloc(fields)"); }
     char *q = c->blob + c->guard * c->esz;
-    const size_t stride = (c->N + kArraySkew) * c->esz;
+    const size_t stride = (c->Np + kArraySkew) * c->esz;
     for (int s = 0; s < 3; s++)
         for (int k = 0; k < 3; k++) { c->E[s][k] = q; q += stride; }
     for (int s = 0; s < 3; s++)
@@ -185,8 +191,8 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     for (int s = 0; s < 2; s++)
         for (int k = 0; k < 3; k++) { c->H[s][k] = q; q += stride; }
     he = hipMemset(c->blob, 0, blob_elems * c->esz);
-    if (he == hipSuccess) he = hipMalloc(&c->ids_base, c->N + 2 * (size_t)c->guard);
-    if (he == hipSuccess) he = hipMemset(c->ids_base, 0, c->N + 2 * (size_t)c->guard);
+    if (he == hipSuccess) he = hipMalloc(&c->ids_base, c->Np + 2 * (size_t)c->guard);
+    if (he == hipSuccess) he = hipMemset(c->ids_base, 0, c->Np + 2 * (size_t)c->guard);
     if (he == hipSuccess) c->ids = c->ids_base + c->guard;
     if (he == hipSuccess) he = hipMalloc(&c->lut, sizeof(MatCoef) * kMaxMaterials);
     const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
@@ -340,10 +346,10 @@ extern "C" int fdtd_ctx_set_field(fdtd_ctx *c, int field, int level, const float
 {
     if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fdtd_fail(FDTD_EINVAL, "bad field/level");
     resolve_scales(c);
-    std::vector<char> buf(c->N * c->esz);
-    fdtd_to_storage(c->storage, host, c->N, field_factor(c, field), buf.data());
+    std::vector<char> buf(c->Np * c->esz);
+    fdtd_to_storage_pitched(c->storage, host, c->nx, (size_t)c->ny * c->nz, c->pitch, field_factor(c, field), buf.data());
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(field_dev(c, field, level), buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(field_dev(c, field, level), buf.data(), buf.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -351,10 +357,10 @@ extern "C" int fdtd_ctx_get_field(fdtd_ctx *c, int field, int level, float *host
 {
     if (!c || !host || field < 0 || field > 8 || level < 0 || level > 2) return fdtd_fail(FDTD_EINVAL, "bad field/level");
     resolve_scales(c);
-    std::vector<char> buf(c->N * c->esz);
+    std::vector<char> buf(c->Np * c->esz);
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(buf.data(), field_dev(c, field, level), c->N * c->esz, hipMemcpyDeviceToHost));
-    fdtd_from_storage(c->storage, buf.data(), c->N, field_factor(c, field), host);
+    HIP_TRY(hipMemcpy(buf.data(), field_dev(c, field, level), buf.size(), hipMemcpyDeviceToHost));
+    fdtd_from_storage_pitched(c->storage, buf.data(), c->nx, (size_t)c->ny * c->nz, c->pitch, field_factor(c, field), host);
     return 0;
 }
 
@@ -484,13 +490,14 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
     if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
     if (c->side) HIP_TRY(hipStreamSynchronize(c->side));
     const int cnt = std::min(max_slices, c->slice_count);
-    const size_t pl = (size_t)c->plane;
+    const size_t pl = (size_t)c->plane, nxy = (size_t)c->nx * c->ny;
     for (int s = 0; s < cnt; s++) {
         if (steps) steps[s] = c->slice_steps[s];
         if (data)
             for (int f = 0; f < 9; f++)
-                fdtd_from_storage(c->storage, c->slice_host + ((size_t)s * 9 + f) * pl * c->esz, pl, field_factor(c, f),
-                             data + ((size_t)s * 9 + f) * pl);
+                for (int j = 0; j < c->ny; j++)
+                    fdtd_from_storage(c->storage, c->slice_host + (((size_t)s * 9 + f) * pl + (size_t)j * c->pitch) * c->esz,
+                                      c->nx, field_factor(c, f), data + ((size_t)s * 9 + f) * nxy + (size_t)j * c->nx);
     }
     return cnt;
 }
@@ -539,5 +546,14 @@ extern "C" int fdtd_ctx_scales(fdtd_ctx *c, double *scale_e, double *scale_d, do
     if (scale_e) *scale_e = c->sE;
     if (scale_d) *scale_d = c->sD;
     if (scale_h) *scale_h = c->sH;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_layout(fdtd_ctx *c, int *pitch, long long *plane, int *elem_bytes)
+{
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
+    if (pitch) *pitch = c->pitch;
+    if (plane) *plane = c->plane;
+    if (elem_bytes) *elem_bytes = (int)c->esz;
     return 0;
 }

@@ -177,13 +177,11 @@ int main(int argc, char **argv)
             if (!opt.quiet) printf("Running iteration %d\n", it + 1);
             CHECK_RC(fdtd_step_fused(ctx, stream));
             if ((it + 1) % opt.every == 0) {
-                std::vector<const float *> dev(9);
                 for (int f = 0; f < 9; f++) {
-                    void *p = nullptr;
-                    CHECK_RC(fdtd_ctx_field_ptr(ctx, f, 0, &p));
-                    dev[f] = (const float *)p;
+                    host[f] = (float *)malloc(bytes);
+                    if (!host[f]) { printf("Couldn't allocate %zu bytes\n", bytes); exit(EXIT_FAILURE); }
+                    CHECK_RC(fdtd_ctx_get_field(ctx, f, 0, host[f]));
                 }
-                take(dev);
                 write_set(params, host.data(), it, opt.lines);
             }
         }

@@ -238,12 +238,18 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
 
     // E^n staging registers of one plane: own cell, -y row (wave 0), +y row
     // (wave TY-1, Ex and Ez), gathered columns (wave 1).
-    float pe[3], pr[3], pq[2], pc[3];
+    // Lanes past the row pitch (the last x tile of a row whose pitch is not a
+    // multiple of 64) load nothing: their values are never used, and their
+    // addresses would fall on the next row's first line.
+    const bool lane_in = i < A.pitch;
+    float pe[3] = {0.f, 0.f, 0.f}, pr[3] = {0.f, 0.f, 0.f}, pq[2] = {0.f, 0.f}, pc[3];
     auto fetch_e = [&](int k) {
         const uint32_t kb = (uint32_t)k * szb;
-        pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
-        if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
-        if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
+        if (lane_in) {
+            pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
+            if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
+            if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
+        }
         if (w_col) { pc[0] = X::ld(ex0, kb + c_col); pc[1] = X::ld(ey0, kb + c_col); pc[2] = X::ld(ez0, kb + c_col); }
     };
     auto stage_e = [&](int buf) {
@@ -260,14 +266,20 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     };
     auto fetch_own = [&](int k, Own &o) {
         const uint32_t ob = (uint32_t)k * szb + c_own;
-        o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
-        o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
-        o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
-        o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
-        o.id = __builtin_nontemporal_load(ids + ob / ES);
-        // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TX, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;
-        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
+        if (lane_in) {
+            o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
+            o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
+            o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
+            o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+            o.id = __builtin_nontemporal_load(ids + ob / ES);
+            // halo H^{n-1/2} of row (i, j0-1): Hx, Hz
+            if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
+        } else {
+            o.hxo = o.hyo = o.hzo = o.dxn = o.dyn = o.dzn = o.dxm = o.dym = o.dzm = o.exm = o.eym = o.ezm = 0.f;
+            o.id = 0;
+        }
+        // halo H^{n-1/2} of column (i0-1 | i0+TX, row cr): Hy, Hz
         if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
     };
 
@@ -281,7 +293,11 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         __syncthreads();
         const int k = k0 - 1;
         const uint32_t ob = (uint32_t)k * szb + c_own;
-        const float hx0 = X::ld(hxo, ob), hy0 = X::ld(hyo, ob);
+        float hx0 = 0.f, hy0 = 0.f;
+        if (lane_in) {
+            hx0 = X::ld(hxo, ob);
+            hy0 = X::ld(hyo, ob);
+        }
         const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
         const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
         const float ux = hx0 - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);

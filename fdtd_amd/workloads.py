@@ -20,6 +20,15 @@ Workloads (sizes as stated):
   large_phantom 320x320x880, 50 tissue ids from seeded ellipsoids and
                 z-cylinders, 10-cell Jz dipole at the phantom edge, 900 MHz,
                 5000 steps.
+  batched_set   64 independent grids, edges nx, ny in [48,160] and nz in
+                [48,224] (48^3 to 160x160x224), each with its own phantom of
+                4-12 Debye materials (nested ellipsoids), a point source at a
+                seeded position, a carrier in [300 MHz, 6 GHz] and 500-3000
+                steps; fp32.
+  served_stream Poisson arrivals (mean 40 per second) of solves with edges
+                in [32,192] per axis, 200-4000 steps, fp32 or fp16 storage
+                (even odds), 4-12 material ellipsoid phantoms and a point or
+                a z-dipole source (even odds).
 """
 import ctypes as C
 import dataclasses
@@ -240,3 +249,81 @@ def large_phantom(steps=5000, seed=12348, shape=(320, 320, 880)):
                         np.int32)
     return Workload("large_phantom", shape, params, ids, table, src, jz[:max(steps, 1)].copy(), probes, steps,
                     slice_k=nz // 2, slice_every=500, seed=seed, freq=9.0e8)
+
+
+# Vacuum cells kept between a phantom and every face.  First-order Mur with
+# the reference's rp (from the default material, i.e. vacuum) diverges when
+# a dielectric reaches the boundary: a Debye ellipsoid touching the faces
+# grows by 10^29 within 1600 steps, one kept 10 cells inside stays bounded
+# (the same holds with sigma = 0 or tau = 0, and without Mur it decays).
+PHANTOM_MARGIN = 10
+
+
+def ellipsoid_phantom(rng, shape, nmat):
+    """Vacuum background plus nmat - 1 seeded ellipsoids painted largest
+    first, inside the box PHANTOM_MARGIN cells in from every face (semi-axes
+    15-45 % of the box edge, centres in the middle 40 %, each ellipsoid
+    clipped to the box)."""
+    nx, ny, nz = shape
+    M = PHANTOM_MARGIN
+    table = np.concatenate([np.asarray([(0.0, 1.0, 1.0, 0.0)], np.float32), debye_table(rng, nmat - 1)])
+    shapes = []
+    for _ in range(nmat - 1):
+        box = [n - 1 - 2 * M for n in shape]
+        a = [rng.uniform(0.15, 0.45) * L for L in box]
+        c = [M + rng.uniform(0.3, 0.7) * L for L in box]
+        a = [min(a[d], c[d] - M, shape[d] - 1 - M - c[d]) for d in range(3)]
+        if min(a) < 1.0:
+            continue
+        shapes.append((a[0] * a[1] * a[2], c, a))
+    shapes.sort(key=lambda t: -t[0])
+    ids = np.zeros((nz, ny, nx), np.uint8)
+    for m, (_, c, a) in enumerate(shapes, start=1):
+        _paint_ellipsoid(ids, c, a, m)
+    return ids, table
+
+
+def _grid(name, rng, shape, steps, freq, nmat, dipole, seed):
+    nx, ny, nz = shape
+    params, jz = derive(shape, steps, freq, 2.0e-3, 0.99)
+    ids, table = ellipsoid_phantom(rng, shape, nmat)
+    x, y = rng.randint(2, nx - 3), rng.randint(2, ny - 3)
+    if dipole and nz >= 14:
+        z0 = rng.randint(1, nz - 12)
+        src = np.asarray([[x, y, z] for z in range(z0, z0 + 10)], np.int32)
+    else:
+        src = np.asarray([[x, y, rng.randint(1, nz - 3)]], np.int32)
+    probes = np.asarray([[8, nx // 2, ny // 2, nz // 2], [8, min(x + 3, nx - 1), y, int(src[0, 2])]], np.int32)
+    return Workload(name, shape, params, ids, table, src, jz[:max(steps, 1)].copy(), probes, steps, seed=seed,
+                    freq=freq)
+
+
+def batched_set(count=64, seed=12349):
+    """count uneven grids between 48^3 and 160x160x224 (see module doc)."""
+    rng = SplitMix64(seed)
+    out = []
+    for q in range(count):
+        shape = (rng.randint(48, 160), rng.randint(48, 160), rng.randint(48, 224))
+        nmat = rng.randint(4, 12)
+        freq = rng.uniform(3.0e8, 6.0e9)
+        steps = rng.randint(500, 3000)
+        out.append(_grid(f"batched_{q}", rng, shape, steps, freq, nmat, False, seed))
+    return out
+
+
+def served_stream(count=120, seed=12350, rate=40.0, f16_share=0.5):
+    """[(arrival seconds, Workload, storage 'f32'|'f16')] of an open-loop
+    Poisson stream (see module doc)."""
+    rng = SplitMix64(seed)
+    t = 0.0
+    out = []
+    for q in range(count):
+        t += -np.log(1.0 - rng.uniform()) / rate
+        shape = (rng.randint(32, 192), rng.randint(32, 192), rng.randint(32, 192))
+        steps = rng.randint(200, 4000)
+        storage = "f16" if rng.uniform() < f16_share else "f32"
+        nmat = rng.randint(4, 12)
+        freq = rng.uniform(3.0e8, 6.0e9)
+        dipole = rng.uniform() < 0.5
+        out.append((t, _grid(f"served_{q}", rng, shape, steps, freq, nmat, dipole, seed), storage))
+    return out

@@ -104,8 +104,13 @@ int fdtd_ctx_set_slices(fdtd_ctx *ctx, int k, int interval, int capacity);
 int fdtd_ctx_set_field(fdtd_ctx *ctx, int field, int level, const float *host);
 int fdtd_ctx_get_field(fdtd_ctx *ctx, int field, int level, float *host);
 /* Device address of a field (for zero-copy callers); elements are float,
- * or 16-bit scaled values under 16-bit storage (see fdtd_ctx_scales). */
+ * or 16-bit scaled values under 16-bit storage (see fdtd_ctx_scales), laid
+ * out as fdtd_ctx_layout says. */
 int fdtd_ctx_field_ptr(fdtd_ctx *ctx, int field, int level, void **dev);
+/* Device layout of every field: cell (i, j, k) is element
+ * k*plane + j*pitch + i; pitch >= nx rounds rows up to 128-byte lines,
+ * plane = pitch*ny; elem_bytes is 4 or 2. */
+int fdtd_ctx_layout(fdtd_ctx *ctx, int *pitch, long long *plane, int *elem_bytes);
 /* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales are 1/32 of
  * the largest per-step D/eps0 kick of the source waveform, fixed at the
  * first step. */
@@ -208,6 +213,17 @@ int fdtd_batch_set_shape(fdtd_batch *b, int tile_y, int kchunk, int graph_steps)
  * their zero initial state.  Every grid must be unstepped. */
 int fdtd_batch_autotune(fdtd_batch *b, int trial_steps, hipStream_t stream, int *tile_y, int *kchunk);
 int fdtd_batch_info(fdtd_batch *b, FdtdBatchInfo *info);
+
+/* One grid solved through the five drop-in entry points of
+ * fdtd_calculations.h, as the reference's main loop drives them
+ * (fdtd.cu:115-530): per-voxel material arrays, the reference's limits (16
+ * sources, the first 512 jz values), fp32 storage.  Allocates, steps
+ * inst->steps times on `stream`, writes nprobe x steps probe rows to
+ * probes_host and frees.  Uses the process-global symbols of
+ * copySymbolsToDevice: calls must not overlap.  The sequential baseline of
+ * batched and served runs. */
+int fdtd_dropin_solve(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst, double *probes_host,
+                      hipStream_t stream);
 
 /* ---- serving ------------------------------------------------------------
  *

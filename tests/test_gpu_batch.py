@@ -142,3 +142,13 @@ def test_batch_rejects_what_does_not_fit():
     with pytest.raises(F.FdtdError, match="-5"):
         b.add(x)
     b.close()
+
+
+def test_dropin_solve_equals_fused():
+    """The drop-in entry points driven like the reference's main loop give the
+    fused context's probe series bit for bit (jz within the 512-value limit)."""
+    for q, jm in enumerate([F.JZ_HOLD3, F.JZ_STEP]):
+        inst = make_case(300 + q, (26, 19, 31), 6, 23, jz_mode=jm, rp=True, nprobe=5, jz_limit=512)
+        got = F.dropin_solve(inst)
+        want = single(inst)[1]
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), f"jz mode {jm}"

@@ -47,6 +47,7 @@ EXPORTS = {
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
+                   "fdtd_ctx_set_energy", "fdtd_energy_read",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
@@ -164,6 +165,8 @@ def lib():
         L.fdtd_ctx_set_timing.argtypes = [vp, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
+        L.fdtd_ctx_set_energy.argtypes = [vp, i, i]
+        L.fdtd_energy_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         PP, PE, PI = C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(FdtdInstance)
         L.fdtd_batch_create.argtypes = [C.c_size_t, C.POINTER(vp)]
@@ -284,6 +287,18 @@ class Context:
 
     def set_slices(self, k, interval, capacity):
         check(lib().fdtd_ctx_set_slices(self._h, k, interval, capacity), "set_slices")
+
+    def set_energy(self, interval, capacity):
+        check(lib().fdtd_ctx_set_energy(self._h, interval, capacity), "set_energy")
+
+    def energy(self, max_rows=100000):
+        """(step counts, fp64 energy sums) recorded so far."""
+        steps = np.zeros(max_rows, np.int32)
+        e = np.zeros(max_rows, np.float64)
+        got = lib().fdtd_energy_read(self._h, _ptr(steps), _ptr(e), max_rows)
+        if got < 0:
+            check(got, "energy_read")
+        return steps[:got], e[:got]
 
     def set(self, field, level, arr):
         fid = FIELDS.index(field) if isinstance(field, str) else field

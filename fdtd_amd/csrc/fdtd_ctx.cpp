@@ -67,6 +67,10 @@ struct fdtd_ctx {
     hipStream_t side = nullptr;
     hipEvent_t slice_ev = nullptr;
 
+    int energy_interval = 0, energy_cap = 0, energy_rows = 0;
+    double *energy_out = nullptr;        // device: energy_cap sums, then kEnergyBlocks partials
+    std::vector<int> energy_steps;
+
     std::vector<hipEvent_t> tev;  // start/end pairs around fused launches
     int tcap = 0, tused = 0;
 
@@ -228,6 +232,7 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
     (void)hipFree(c->probes);
     (void)hipFree(c->probe_out);
     (void)hipFree(c->slice_stage);
+    (void)hipFree(c->energy_out);
     if (c->slice_host) (void)hipHostFree(c->slice_host);
     if (c->slice_ev) (void)hipEventDestroy(c->slice_ev);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -447,6 +452,23 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
         e = launch_probes(p, c->storage, stream);
         if (e != hipSuccess) return fdtd_hip_fail(e, "probe_cells");
     }
+    if (c->energy_interval > 0 && c->n % c->energy_interval == 0 && c->energy_rows < c->energy_cap) {
+        EnergyArgs g;
+        for (int q = 0; q < 3; q++) {
+            g.D[q] = field_dev(c, q, 0);
+            g.H[q] = field_dev(c, 3 + q, 0);
+            g.E[q] = field_dev(c, 6 + q, 0);
+        }
+        g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.pitch = c->pitch; g.plane = c->plane;
+        g.ed_factor = field_factor(c, FDTD_EX) * field_factor(c, FDTD_DX);
+        g.hh_factor = (double)c->mu0 * field_factor(c, FDTD_HX) * field_factor(c, FDTD_HX);
+        g.partial = c->energy_out + c->energy_cap;
+        g.out = c->energy_out + c->energy_rows;
+        e = launch_energy(g, c->storage, stream);
+        if (e != hipSuccess) return fdtd_hip_fail(e, "energy");
+        c->energy_steps.push_back(c->n);
+        c->energy_rows++;
+    }
     if (c->slice_interval > 0 && c->n % c->slice_interval == 0 && c->slice_count < c->slice_cap) {
         const size_t pb = (size_t)c->plane * c->esz;
         char *stage = c->slice_stage + (size_t)c->slice_count * 9 * pb;
@@ -556,4 +578,32 @@ extern "C" int fdtd_ctx_layout(fdtd_ctx *c, int *pitch, long long *plane, int *e
     if (plane) *plane = c->plane;
     if (elem_bytes) *elem_bytes = (int)c->esz;
     return 0;
+}
+
+extern "C" int fdtd_ctx_set_energy(fdtd_ctx *c, int interval, int capacity)
+{
+    if (!c || interval < 0 || capacity < 0) return fdtd_fail(FDTD_EINVAL, "bad energy series");
+    HIP_TRY(hipDeviceSynchronize());
+    (void)hipFree(c->energy_out);
+    c->energy_out = nullptr;
+    c->energy_interval = interval;
+    c->energy_cap = capacity;
+    c->energy_rows = 0;
+    c->energy_steps.clear();
+    if (interval == 0 || capacity == 0) {
+        c->energy_interval = 0;
+        return 0;
+    }
+    HIP_TRY(hipMalloc(&c->energy_out, sizeof(double) * ((size_t)capacity + kEnergyBlocks)));
+    return 0;
+}
+
+extern "C" int fdtd_energy_read(fdtd_ctx *c, int *steps, double *energy, int max_rows)
+{
+    if (!c || max_rows < 0) return fdtd_fail(FDTD_EINVAL, "bad energy read");
+    HIP_TRY(hipDeviceSynchronize());
+    const int rows = std::min(max_rows, c->energy_rows);
+    if (rows > 0 && energy) HIP_TRY(hipMemcpy(energy, c->energy_out, sizeof(double) * rows, hipMemcpyDeviceToHost));
+    for (int r = 0; r < rows && steps; r++) steps[r] = c->energy_steps[r];
+    return rows;
 }

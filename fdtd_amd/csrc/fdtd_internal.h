@@ -98,6 +98,20 @@ struct ProbeArgs {
     int row;
 };
 
+// Electromagnetic energy sum_cells (E.D / 2 + mu0 |H|^2 / 2) of the newest
+// levels, accumulated in fp64: per-block partial sums, then one block adds
+// them in a fixed order (the result does not depend on scheduling).
+struct EnergyArgs {
+    const void *E[3], *D[3], *H[3];
+    int nx, ny, nz, pitch;
+    long long plane;
+    double ed_factor, hh_factor;         // stored products -> SI: E.D and mu0 H^2
+    double *partial;                     // kEnergyBlocks doubles
+    double *out;                         // the sum is written here
+};
+constexpr int kEnergyBlocks = 1024;
+hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
+
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);

@@ -665,6 +665,57 @@ __global__ void batch_advance(BatchShared *sh, int steps)
     if (threadIdx.x == 0) sh->step += steps;
 }
 
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Block-wide sum in a fixed order (lane tree within waves, then waves in
+// order); the result is valid in thread 0.
+template <int NT>
+__device__ inline double block_sum(double v)
+{
+    __shared__ double part[NT / 64];
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) part[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NT / 64; q++) t += part[q];
+    return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void energy_partial(const EnergyArgs A)
+{
+    using X = Elem<T>;
+    const long long nxy = (long long)A.nx * A.ny, n = nxy * A.nz;
+    double ed = 0.0, hh = 0.0;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        const int k = (int)(q / nxy);
+        const long long r = q - (long long)k * nxy;
+        const int j = (int)(r / A.nx), i = (int)(r - (long long)j * A.nx);
+        const uint32_t b = (uint32_t)(((long long)k * A.plane + (long long)j * A.pitch + i) * (long long)sizeof(T));
+        for (int c = 0; c < 3; c++) {
+            const double e = X::ld((const T *)A.E[c], b), d = X::ld((const T *)A.D[c], b), h = X::ld((const T *)A.H[c], b);
+            ed += e * d;
+            hh += h * h;
+        }
+    }
+    const double t = block_sum<256>(0.5 * ed * A.ed_factor + 0.5 * hh * A.hh_factor);
+    if (threadIdx.x == 0) A.partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void energy_final(const EnergyArgs A)
+{
+    double v = 0.0;
+    for (int q = threadIdx.x; q < kEnergyBlocks; q += 256) v += A.partial[q];
+    const double t = block_sum<256>(v);
+    if (threadIdx.x == 0) *A.out = t;
+}
+
 template <typename T>
 __global__ void probe_cells(const ProbeArgs P)
 {
@@ -701,6 +752,19 @@ hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s)
     case kStoreBF16: mur_faces<__hip_bfloat16><<<grid, kMurBlock, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s)
+{
+    switch (storage) {
+    case kStoreF32:
+    case kStoreF32Fast: energy_partial<float><<<kEnergyBlocks, 256, 0, s>>>(a); break;
+    case kStoreF16: energy_partial<__half><<<kEnergyBlocks, 256, 0, s>>>(a); break;
+    case kStoreBF16: energy_partial<__hip_bfloat16><<<kEnergyBlocks, 256, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    energy_final<<<1, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -99,6 +99,16 @@ int fdtd_ctx_set_probes(fdtd_ctx *ctx, int nprobe, const int *probes, int capaci
  * host memory on a side stream; up to `capacity` slices are kept. */
 int fdtd_ctx_set_slices(fdtd_ctx *ctx, int k, int interval, int capacity);
 
+/* Record the field energy sum(E.D/2 + mu0 |H|^2/2) over all cells, of the
+ * newest levels, every `interval` steps (up to `capacity` values),
+ * accumulated in fp64 on the device in a fixed order. */
+int fdtd_ctx_set_energy(fdtd_ctx *ctx, int interval, int capacity);
+/* Waits for the device; returns the number of values and fills steps[]
+ * (the step count after which each was taken) and energy[] (the sum of the
+ * cell energy densities, J/m^3; times dx*dy*dz for joules), up to
+ * max_rows. */
+int fdtd_energy_read(fdtd_ctx *ctx, int *steps, double *energy, int max_rows);
+
 /* level 0 = newest (after the last step), 1 = the one before, 2 = oldest
  * ring slot.  H has one level. */
 int fdtd_ctx_set_field(fdtd_ctx *ctx, int field, int level, const float *host);

@@ -1,0 +1,58 @@
+"""FP64 energy accumulation on the device: sum over cells of E.D/2 +
+mu0 |H|^2/2 of the newest levels.  With fp32 storage the fields are the
+oracle's bit for bit, so the device sum equals the oracle's FP64 sum of the
+same fields up to the summation order (bound 1e-12 relative)."""
+import numpy as np
+import pytest
+
+import fdtd_amd as F
+from tests.helpers import EPS0, MU0, oracle_sim, random_materials, random_sources
+
+pytestmark = pytest.mark.gpu
+
+DT = 3.8105118e-12
+DX = 2.0e-3
+
+
+def _jz(n=40):
+    t = np.arange(n, dtype=np.float64)
+    return (-(t - 12) / 4 * np.exp(-((t - 12) / 4) ** 2)).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(30, 22, 26), (65, 9, 17)])
+def test_energy_series_matches_oracle(shape):
+    rng = np.random.default_rng(11)
+    ids, table = random_materials(rng, shape, 5)
+    src = random_sources(rng, shape, 3)
+    jz = _jz()
+    ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0))
+    ctx.set_materials(ids, table)
+    ctx.set_sources(src, jz)
+    ctx.set_energy(4, 100)
+    ctx.run(40)
+    steps, energy = ctx.energy()
+    assert list(steps) == list(range(4, 41, 4))
+    s = oracle_sim(shape, DT, DX)
+    t = table[ids.astype(np.int64)]
+    s.set_materials(t[..., 0].copy(), t[..., 1].copy(), t[..., 2].copy(), t[..., 3].copy())
+    s.set_sources(src, jz, jz_mode=1)
+    want = []
+    for _ in range(10):
+        s.step(4)
+        want.append(s.energy())
+    want = np.asarray(want)
+    assert (want > 0).all()
+    np.testing.assert_allclose(energy, want, rtol=1e-12, atol=0)
+
+
+def test_energy_decays_after_the_pulse_leaves():
+    """Vacuum box with Mur faces: once the pulse has left through the
+    absorbing faces the stored energy falls well below its peak."""
+    shape = (40, 40, 40)
+    ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0))
+    ctx.set_sources(np.asarray([[20, 20, 20]], np.int32), _jz())
+    ctx.set_energy(5, 100)
+    ctx.run(300)
+    _, e = ctx.energy()
+    assert np.isfinite(e).all()
+    assert e[-1] < 0.05 * e.max()

@@ -18,6 +18,9 @@ Two more workloads measure the systems patterns:
                       persistent fused launch per step from captured graphs,
                       shape picked by the autotuner; whole-job Gcell-updates/s
                       beside running the same grids one after another.
+  --workload riken    the RIKEN voxel phantom at body scale (297x522x1719 with
+                      its vacuum pad), from `--phantom` (a PGM directory or
+                      ids packed by `python -m fdtd_amd.phantom`).
   --workload served   an open-loop Poisson stream of solves (served_stream)
                       through the continuous-batching server: solves/s and
                       p50/p99 latency beside the sequential drop-in path (the
@@ -45,7 +48,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plain_256",
-                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served"])
+                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken"])
+    ap.add_argument("--phantom", default="data/riken_ids.npz",
+                    help="riken: PGM slice directory or ids packed by `python -m fdtd_amd.phantom`")
+    ap.add_argument("--specs", default=os.path.join(ROOT, "tests", "golden", "mat_specs_riken"),
+                    help="riken: tissue table")
     ap.add_argument("--requests", type=int, default=120, help="served: requests of the stream")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served: device memory budget")
     ap.add_argument("--slots", type=int, default=64, help="served: grids on the device at once")
@@ -62,8 +69,13 @@ def parse():
     return ap.parse_args()
 
 
-def build_workload(name, steps):
+def build_workload(name, steps, a=None):
     from fdtd_amd import workloads as W
+    if name == "riken":
+        import fdtd_amd as F
+        from fdtd_amd import phantom as P
+        table, _ = F.load_tissue_table(a.specs)
+        return W.riken(P.load(a.phantom, P.RIKEN_SHAPE), table, steps=steps)
     if name == "plain_256":
         return W.plain_256(steps=steps)
     if name == "large_phantom":
@@ -112,7 +124,7 @@ def main():
         return bench_served(a)
 
     total = a.warmup + a.steps
-    w = build_workload(a.workload, total)
+    w = build_workload(a.workload, total, a)
     nx, ny, nz = w.shape
     cells = nx * ny * nz
 
@@ -160,7 +172,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (f64 Debye sums)" if a.storage == "f32" else f"{a.storage} storage, f32 arithmetic",
-        "data": "synthetic (seeded phantom)",
+        "data": ("RIKEN voxel phantom (the reference's data/pgmdata), vacuum-padded" if w.name == "riken"
+                 else "synthetic (seeded phantom)"),
         "config": {"workload": w.name, "grid": [nx, ny, nz], "cells": cells, "materials": int(w.table.shape[0]),
                    "sources": int(w.sources.shape[0]), "probes": int(w.probes.shape[0]),
                    "slice_every": w.slice_every, "launch": shape, "parallelism": "single GPU"},

@@ -51,11 +51,12 @@ EXPORTS = {
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
-                   "fdtd_dropin_solve",
+                   "fdtd_dropin_solve", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
 ENOMEM = -5
+PGM_REFERENCE, PGM_IMAGE = 0, 1
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
@@ -182,6 +183,8 @@ def lib():
         L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
         L.fdtd_dropin_solve.argtypes = [PP, PE, PI, vp, vp]
+        L.fdtd_load_tissue_table.argtypes = [C.c_char_p, vp]
+        L.fdtd_load_phantom_ids.argtypes = [C.c_char_p, i, i, i, i, i, vp]
         L.fdtd_serve_create.argtypes = [C.POINTER(FdtdServeConfig), C.POINTER(vp)]
         L.fdtd_serve_destroy.argtypes = [vp]
         L.fdtd_serve_submit.argtypes = [vp, PP, PE, PI, C.POINTER(C.c_longlong)]
@@ -420,6 +423,25 @@ class Instance:
         check(lib().fdtd_batch_instance_bytes(C.byref(self.params), C.byref(self.ext), C.byref(self.c),
                                               C.byref(b)), "instance_bytes")
         return b.value
+
+
+def load_tissue_table(path):
+    """95 x {sigma, eps_s, eps_inf, tau} rows (row = tissue code) and the rows read."""
+    t = np.zeros((95, 4), np.float32)
+    rows = lib().fdtd_load_tissue_table(str(path).encode(), _ptr(t))
+    if rows < 0:
+        check(rows, "load_tissue_table")
+    return t, rows
+
+
+def load_phantom_ids(directory, shape, mode=PGM_REFERENCE, threads=0):
+    """uint8 tissue codes (nz, ny, nx) of <directory>/v1_%05d.pgm."""
+    nx, ny, nz = shape
+    ids = np.zeros((nz, ny, nx), np.uint8)
+    rc = lib().fdtd_load_phantom_ids(str(directory).encode(), nx, ny, nz, mode, threads, _ptr(ids))
+    if rc < 0:
+        check(rc, "load_phantom_ids")
+    return ids
 
 
 def dropin_solve(inst, stream=None):

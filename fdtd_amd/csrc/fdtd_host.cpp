@@ -18,6 +18,7 @@
 
 #include "fdtd.h"
 #include "fdtd_amd.h"
+#include "fdtd_internal.h"
 
 #define HCHECK(call)                                                            \
     do {                                                                        \
@@ -324,67 +325,26 @@ extern "C" void setupMurBoundary(FdtdParams *params, FdtdField *field)
 // fdtd.cu:875-939.  Tissue rows "code name sigma eps_s eps_inf tau" are read
 // until a code >= 94 (stored, then reading stops) or 94 rows; then nz slices
 // <materialsPath>/v1_%05d.pgm, each a 6-token header followed by nx*ny codes
-// read in order whatever the image width (the reference's shear).
+// read in order whatever the image width (the reference's shear).  The
+// parsers are those of fdtd_load_tissue_table / fdtd_load_phantom_ids.
 extern "C" void loadMaterials(FdtdParams *params, FdtdField *field, const char *specsFilePath,
                               const char *materialsPath)
 {
     std::vector<float> specs(95 * 4, 0.0f);
-    std::vector<char> known(95, 0);
-    FILE *sf = fopen(specsFilePath, "r");
-    if (!sf) die("Couldn't open file %s\n", specsFilePath);
-    for (int r = 0; r < 94; r++) {
-        std::string tok[6];
-        bool ok = true;
-        for (int q = 0; q < 6 && ok; q++) ok = token(sf, tok[q]);
-        if (!ok) break;
-        char *end = nullptr;
-        const long code = strtol(tok[0].c_str(), &end, 10);
-        if (*end != '\0') break;  // a comment line ends the table
-        if (code < 0 || code > 94) {
-            fclose(sf);
-            die("Tissue code out of range in %s\n", specsFilePath);
-        }
-        for (int q = 0; q < 4; q++) specs[code * 4 + q] = strtof(tok[2 + q].c_str(), nullptr);
-        known[code] = 1;
-        if (code >= 94) break;
-    }
-    fclose(sf);
-
+    std::string err;
+    if (phantom_read_table(specsFilePath, specs.data(), err) < 0) die("%s\n", err.c_str());
     const int nx = params->nx, ny = params->ny, nz = params->nz;
     const size_t nxy = (size_t)nx * ny;
+    std::vector<unsigned char> ids(nxy);
     std::vector<char> buf;
     for (int iz = 0; iz < nz; iz++) {
         char name[1100];
         snprintf(name, sizeof name, "%s/v1_%05d.pgm", materialsPath, iz + 1);
-        FILE *mf = fopen(name, "r");
-        if (!mf) die("Couldn't open file %s\n", name);
-        fseek(mf, 0, SEEK_END);
-        const long len = ftell(mf);
-        fseek(mf, 0, SEEK_SET);
-        buf.resize((size_t)len + 1);
-        const size_t got = fread(buf.data(), 1, (size_t)len, mf);
-        fclose(mf);
-        buf[got] = '\0';
-        const char *c = buf.data();
-        auto next_token = [&](const char *&s, const char *&e) {
-            while (*c && isspace((unsigned char)*c)) c++;
-            s = c;
-            while (*c && !isspace((unsigned char)*c)) c++;
-            e = c;
-            return s != e;
-        };
-        const char *s, *e;
-        for (int q = 0; q < 6; q++)
-            if (!next_token(s, e)) die("Malformed phantom slice %s\n", name);
+        if (phantom_read_slice(name, nx, ny, FDTD_PGM_REFERENCE, ids.data(), buf, err) < 0) die("%s\n", err.c_str());
         float *sg = field->sigma + iz * nxy, *es = field->epsS + iz * nxy, *ei = field->epsI + iz * nxy,
               *tau = field->tauD + iz * nxy;
         for (size_t t = 0; t < nxy; t++) {
-            while (*c && isspace((unsigned char)*c)) c++;
-            char *endp = nullptr;
-            const long code = strtol(c, &endp, 10);
-            if (endp == c) die("Short phantom slice %s\n", name);
-            c = endp;
-            if (code < 0 || code > 94) die("Tissue code out of range in %s\n", name);
+            const int code = ids[t];
             sg[t] = specs[code * 4 + 0];
             es[t] = specs[code * 4 + 1];
             ei[t] = specs[code * 4 + 2];

@@ -220,6 +220,16 @@ void fdtd_to_storage_pitched(int storage, const float *in, int nx, size_t rows, 
 void fdtd_from_storage_pitched(int storage, const void *in, int nx, size_t rows, int pitch, double factor, float *out);
 void fdtd_pitch_bytes(const unsigned char *in, int nx, size_t rows, int pitch, unsigned char *out);
 int fdtd_check_ext(const FdtdExt &e);
+
+// Phantom input (fdtd_phantom.cpp).  phantom_read_table fills a 95-row
+// {sigma, eps_s, eps_inf, tau} table and returns the rows read;
+// phantom_read_slice fills nx*ny ids of one PGM slice and returns the
+// largest code; both return a negative FDTD_E* code with a message in err.
+#include <string>
+#include <vector>
+int phantom_read_table(const char *path, float *specs, std::string &err);
+int phantom_read_slice(const char *path, int nx, int ny, int mode, unsigned char *ids, std::vector<char> &buf,
+                       std::string &err);
 int fdtd_check_sources(int nx, int ny, int nz, int nsrc, const int *xyz);
 int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes);
 

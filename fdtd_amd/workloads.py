@@ -25,6 +25,12 @@ Workloads (sizes as stated):
                 4-12 Debye materials (nested ellipsoids), a point source at a
                 seeded position, a carrier in [300 MHz, 6 GHz] and 500-3000
                 steps; fp32.
+  riken         the RIKEN voxel phantom (265x490x1687, 53 tissues, tissue
+                table data/mat_specs_riken) padded by 16 vacuum cells per
+                side, with the reference's own source settings
+                (data/input_params: 3 GHz, pulse width 0.5, no carrier
+                modulation, 100 cells per wavelength, so 1 mm cells) at
+                Courant 0.99, a soft Jz source at the grid centre.
   served_stream Poisson arrivals (mean 40 per second) of solves with edges
                 in [32,192] per axis, 200-4000 steps, fp32 or fp16 storage
                 (even odds), 4-12 material ellipsoid phantoms and a point or
@@ -327,3 +333,48 @@ def served_stream(count=120, seed=12350, rate=40.0, f16_share=0.5):
         dipole = rng.uniform() < 0.5
         out.append((t, _grid(f"served_{q}", rng, shape, steps, freq, nmat, dipole, seed), storage))
     return out
+
+
+def riken(ids, table95, steps=200, pad=16):
+    """The RIKEN phantom (uint8 tissue ids (nz, ny, nx), id = tissue code)
+    padded with `pad` vacuum cells on every side; see the module doc."""
+    nz0, ny0, nx0 = ids.shape
+    shape = (nx0 + 2 * pad, ny0 + 2 * pad, nz0 + 2 * pad)
+    nx, ny, nz = shape
+    params, jz = derive_reference(shape, steps)
+    full = np.zeros((nz, ny, nx), np.uint8)
+    full[pad:pad + nz0, pad:pad + ny0, pad:pad + nx0] = ids
+    table = np.asarray(table95, np.float32).copy()
+    table[0] = (0.0, 1.0, 1.0, 0.0)   # code 0 is air
+    c = (nx // 2, ny // 2, nz // 2)
+    src = np.asarray([c], np.int32)
+    probes = np.asarray([[8, c[0], c[1], c[2]], [8, c[0], c[1] - 40, c[2]], [6, c[0], c[1], c[2] + 200]], np.int32)
+    return Workload("riken", shape, params, full, table, src, jz[:max(steps, 1)].copy(), probes, steps, freq=3.0e9)
+
+
+def derive_reference(shape, steps):
+    """Parameters of data/input_params (3 GHz, pulse width 0.5, modulation 0,
+    100 cells per wavelength) on another grid, at Courant 0.99."""
+    L = lib()
+    nx, ny, nz = shape
+    text = params_text(shape, steps, 3.0e9, 0.0, 0.99, pulse_width=0.5, pmf=0.0,
+                       sources=((nx // 2, ny // 2, nz // 2),))
+    text = text.replace("cell_size: 0.000000000e+00\n", "")
+    with tempfile.NamedTemporaryFile("w", suffix=".params", delete=False) as f:
+        f.write(text)
+        path = f.name
+    try:
+        pp = L.initParamsWithPath(path.encode())
+        L.setupSources(pp)
+        p = pp.contents
+        jz = np.ctypeslib.as_array(p.jz, shape=(p.jzCount,)).astype(np.float32).copy()
+        out = FdtdParams()
+        C.memmove(C.byref(out), C.byref(p), C.sizeof(FdtdParams))
+        out.inputPath = out.outputPath = None
+        out.sources = None
+        out.jz = None
+        out.sourcesCount = out.jzCount = 0
+        L.deallocParams(pp)
+    finally:
+        os.unlink(path)
+    return out, jz

@@ -57,7 +57,7 @@ typedef struct
     int jz_index_mode;  /* FDTD_JZ_STEP (default) or FDTD_JZ_HOLD3 */
     int jz_limit;       /* jz entries used; 0 = the whole table, 512 = the reference's device table */
     int tile_y;         /* rows per fused tile: 0 = auto (4), else 2, 4 or 8 */
-    int kchunk;         /* z planes per fused block: 0 = auto (8) */
+    int kchunk;         /* z planes per fused block: 0 = auto (4) */
     int xcd_remap;      /* block -> tile order: 0 identity (default), 1 one contiguous run
                            per XCD, 2 adjacent z-chunk layers on the 8 XCDs */
     int storage;        /* FDTD_STORE_F32 (default), _F16, _BF16 or _F32_FAST */
@@ -148,6 +148,21 @@ int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, 
  * fdtd_ctx_kernel_time waits for them and returns the summed duration. */
 int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
+
+/* ---- voxel phantoms (fdtd.cu:875-939) ------------------------------------ */
+
+#define FDTD_PGM_REFERENCE 0  /* nx*ny codes in file order whatever the image width (fdtd.cu:920-933) */
+#define FDTD_PGM_IMAGE     1  /* the code of image pixel (ix, iy); cells outside the image are air (0) */
+
+/* The tissue table as the reference reads it (fdtd.cu:887-904): table95
+ * receives 95 rows {sigma, eps_s, eps_inf, tau_d}, row = tissue code.
+ * Returns the rows read. */
+int fdtd_load_tissue_table(const char *specs_path, float *table95);
+/* Tissue codes of slices <dir>/v1_%05d.pgm (k+1 for plane k) into ids
+ * (nx*ny*nz bytes, x fastest; id = tissue code, 0..94), parsed on `threads`
+ * threads (0 = all cores).  Use with fdtd_ctx_set_materials(ctx, ids, 95,
+ * table95).  Returns the largest code seen. */
+int fdtd_load_phantom_ids(const char *dir, int nx, int ny, int nz, int mode, int threads, unsigned char *ids);
 
 /* ---- batches of independent grids --------------------------------------
  *

@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plain_256",
                     choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken"])
-    ap.add_argument("--phantom", default="data/riken_ids.npz",
+    ap.add_argument("--phantom", default=os.path.join(ROOT, "data", "riken_ids.npz"),
                     help="riken: PGM slice directory or ids packed by `python -m fdtd_amd.phantom`")
     ap.add_argument("--specs", default=os.path.join(ROOT, "tests", "golden", "mat_specs_riken"),
                     help="riken: tissue table")

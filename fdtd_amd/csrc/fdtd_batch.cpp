@@ -37,7 +37,7 @@ struct Layout {
     int pitch;
     long long plane;
     long long guard;
-    size_t blob, blob_bytes, ids, lut, jz, rp[6], probes, out, out_bytes, total;
+    size_t blob, blob_bytes, ids, lut, jz, rp[6], murx, probes, out, out_bytes, total;
     int jz_n;
 };
 
@@ -70,6 +70,8 @@ Layout layout_of(int nx, int ny, int nz, int storage, int nmat, int jz_n, int np
         L.rp[r] = o;
         o = round_up(o + rps[r] * sizeof(float), 256);
     }
+    L.murx = o;
+    o = round_up(o + 2 * (size_t)ny * nz * sizeof(float4), 256);
     L.probes = o;
     o = round_up(o + (size_t)nprobe * 4 * sizeof(int), 256);
     L.out = o;
@@ -450,7 +452,16 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     const size_t rps[6] = {(size_t)2 * ny * nz, (size_t)nx * 2 * nz, (size_t)nx * ny * 2,
                            (size_t)2 * ny * nz, (size_t)nx * 2 * nz, (size_t)nx * ny * 2};
     std::vector<float> ones;
+    // x-face Mur coefficients from the rp planes (or vacuum)
+    std::vector<float4> murx(2 * (size_t)ny * nz);
+    {
+        std::vector<float> one_x((size_t)2 * ny * nz, 1.0f);
+        mur_x_coefs(ny, nz, p->dt, p->dx, p->mu0, p->eps0, in->rp[0] ? in->rp[0] : one_x.data(),
+                    in->rp[3] ? in->rp[3] : one_x.data(), murx.data(), murx.data() + (size_t)ny * nz);
+    }
     hipError_t he = hipMemsetAsync(base, 0, L.ids + L.Np + 2 * (size_t)L.guard, b->up);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(base + L.murx, murx.data(), sizeof(float4) * murx.size(), hipMemcpyHostToDevice, b->up);
     if (he == hipSuccess) he = hipMemsetAsync(base + L.out, 0, L.out_bytes, b->up);
     if (he == hipSuccess) he = hipMemcpyAsync(base + L.ids + L.guard, ids.data(), L.Np, hipMemcpyHostToDevice, b->up);
     if (he == hipSuccess) he = hipMemcpyAsync(base + L.lut, lut.data(), sizeof(MatCoef) * in->nmat, hipMemcpyHostToDevice, b->up);
@@ -503,7 +514,10 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     d.nprobe = in->nprobe;
     d.probe_rows = in->steps;
     d.dt = p->dt; d.hx = p->dx; d.hy = p->dy; d.hz = p->dz; d.mu0 = p->mu0; d.eps0 = p->eps0;
-    mur_segments(nx, ny, nz, d.mur_seg);
+    d.mur_x = fused_mur_x(nx);
+    d.murx0 = (const float4 *)(base + L.murx);
+    d.murx1 = d.murx0 + (size_t)ny * nz;
+    mur_segments(nx, ny, nz, d.mur_seg, mur_skip_mask(d.mur_x));
     I.start = b->step;
     I.used = true;
     b->dirty = true;

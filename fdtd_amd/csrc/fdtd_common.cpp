@@ -289,3 +289,23 @@ void fdtd_pitch_bytes(const unsigned char *in, int nx, size_t rows, int pitch, u
         memset(out + r * pitch + nx, 0, pitch - nx);
     }
 }
+
+void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, const float *rpx0, const float *rpxEnd,
+                 float4 *out0, float4 *out1)
+{
+    auto one = [&](float rpf, float rpn) {
+        const float sf = std::sqrt(mu0 * eps0 * rpf), sn = std::sqrt(mu0 * eps0 * rpn);
+        float4 c;
+        c.x = 1 / (dt + hx * sf);
+        c.y = dt - hx * sn;
+        c.z = dt + hx * sn;
+        c.w = dt - hx * sf;
+        return c;
+    };
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+            const size_t r = ((size_t)k * ny + j) * 2, q = (size_t)k * ny + j;
+            out0[q] = one(rpx0[r], rpx0[r + 1]);
+            out1[q] = one(rpxEnd[r + 1], rpxEnd[r]);
+        }
+}

@@ -50,6 +50,7 @@ struct fdtd_ctx {
     std::vector<float> table;             // nmat x {sigma, eps_s, eps_inf, tau}
     bool lut_dirty = true;
     float *rp[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float4 *murx = nullptr;              // x-face Mur coefficients: 2 planes of ny*nz
     std::vector<unsigned char> ids_host;  // material ids without source bits
     std::vector<int> src;                 // 3 per source
     std::vector<float> jz;
@@ -128,6 +129,14 @@ int upload_lut(fdtd_ctx *c)
     HIP_TRY(hipMemcpy(c->lut, lut.data(), sizeof(MatCoef) * c->nmat, hipMemcpyHostToDevice));
     c->lut_dirty = false;
     return 0;
+}
+
+hipError_t upload_murx(fdtd_ctx *c, const float *rpx0, const float *rpxEnd)
+{
+    const size_t n = (size_t)c->ny * c->nz;
+    std::vector<float4> h(2 * n);
+    mur_x_coefs(c->ny, c->nz, c->dt, c->dx, c->mu0, c->eps0, rpx0, rpxEnd, h.data(), h.data() + n);
+    return hipMemcpy(c->murx, h.data(), sizeof(float4) * 2 * n, hipMemcpyHostToDevice);
 }
 
 }  // namespace
@@ -210,6 +219,11 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
             he = hipMemcpy(c->rp[r], ones.data(), rps[r] * sizeof(float), hipMemcpyHostToDevice);
         }
     }
+    if (he == hipSuccess) he = hipMalloc(&c->murx, sizeof(float4) * 2 * (size_t)c->ny * c->nz);
+    if (he == hipSuccess) {
+        std::vector<float> o0((size_t)2 * c->ny * c->nz, 1.0f);
+        he = upload_murx(c, o0.data(), o0.data());
+    }
     if (he != hipSuccess) {
         int rc = fdtd_hip_fail(he, "fdtd_ctx_create");
         fdtd_ctx_destroy(c);
@@ -231,6 +245,7 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
     (void)hipFree(c->ids_base);
     (void)hipFree(c->lut);
     for (int r = 0; r < 6; r++) (void)hipFree(c->rp[r]);
+    (void)hipFree(c->murx);
     (void)hipFree(c->probes);
     (void)hipFree(c->probe_out);
     (void)hipFree(c->slice_stage);
@@ -293,6 +308,7 @@ extern "C" int fdtd_ctx_set_rp(fdtd_ctx *c, const float *rpx0, const float *rpy0
         if (!src[r]) return fdtd_fail(FDTD_EINVAL, "null rp plane %d", r);
         HIP_TRY(hipMemcpy(c->rp[r], src[r], rps[r] * sizeof(float), hipMemcpyHostToDevice));
     }
+    HIP_TRY(upload_murx(c, rpx0, rpxEnd));
     return 0;
 }
 
@@ -416,6 +432,9 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
     a.xcd_remap = c->ext.xcd_remap;
     a.guard = (unsigned)c->guard;
+    a.mur_x = fused_mur_x(c->nx);
+    a.murx0 = c->murx;
+    a.murx1 = c->murx + (size_t)c->ny * c->nz;
     const bool timed = c->tused < c->tcap;
     if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
     hipError_t e = launch_fused_step(a, c->ty, c->storage, stream);
@@ -432,7 +451,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     m.rpxEnd = c->rp[3]; m.rpyEnd = c->rp[4]; m.rpzEnd = c->rp[5];
     m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pitch = c->pitch; m.plane = c->plane;
     m.dt = c->dt; m.hx = c->dx; m.hy = c->dy; m.hz = c->dz; m.mu0 = c->mu0; m.eps0 = c->eps0;
-    mur_segments(c->nx, c->ny, c->nz, m.seg_start);
+    mur_segments(c->nx, c->ny, c->nz, m.seg_start, mur_skip_mask(a.mur_x));
     e = launch_mur(m, c->storage, stream);
     if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces");
 

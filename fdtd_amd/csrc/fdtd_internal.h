@@ -58,7 +58,33 @@ struct StepArgs {
     int tiles_x, tiles_y, tiles_z;  // logical tile grid
     int xcd_remap;                  // 0: identity, 1: contiguous runs per XCD, 2: layer-interleaved
     unsigned guard;                 // addressable elements before and after every field array (and ids)
+    // Mur x faces updated inside the fused kernel (fused_mur_x): 0 none, 1
+    // all four (Ey, Ez at i = 0 and nx-1), 2 all but Ez at i = nx-1.
+    int mur_x;
+    const float4 *murx0, *murx1;    // per (k, j) coefficients of the x-low / x-high faces (mur_x_coefs)
 };
+
+// Coefficients {q, A, B, C} of the first-order Mur update of one x-face
+// cell, E' = q*((A*tn + B*sn) - C*sf), evaluated in float exactly as the
+// reference's expression (fdtd_calculations.cu:303-329, 369-395) rounds
+// them: q = 1/(dt + h sqrt(mu0 eps0 rp_face)), A = dt - h sqrt(mu0 eps0
+// rp_nb), B = dt + h sqrt(mu0 eps0 rp_nb), C = dt - h sqrt(mu0 eps0
+// rp_face).  Planes of ny*nz entries, (k, j) order: out0 from rpx0 (face
+// index 0), out1 from rpxEnd (face index 1).
+void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, const float *rpx0, const float *rpxEnd,
+                 float4 *out0, float4 *out1);
+
+// Which Mur x faces the fused kernel takes over: the x-high Ez face needs
+// E^{n+1}_z at i = nx-2 from the neighbouring lane, which lives in the
+// previous tile when nx-1 is a multiple of 64; that face then stays in the
+// Mur kernel.
+inline int fused_mur_x(int nx) { return (nx - 1) % 64 == 0 ? 2 : 1; }
+// Mur segments (in mur_segments order) left to the Mur kernel.
+inline unsigned mur_skip_mask(int mur_x)
+{
+    // segments 4, 5: Ey at i = 0, nx-1; 8, 9: Ez at i = 0, nx-1
+    return mur_x == 1 ? (1u << 4 | 1u << 5 | 1u << 8 | 1u << 9) : mur_x == 2 ? (1u << 4 | 1u << 5 | 1u << 8) : 0u;
+}
 
 // Row pitch (elements) of the fused layout: rows start on 128-byte lines, so
 // the 256-byte row segment a wave64 of fp32 lanes reads (or a 128-byte one
@@ -115,7 +141,9 @@ hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
-void mur_segments(int nx, int ny, int nz, int *seg_start);
+// Prefix sums of the 12 Mur face segments; segments whose bit is set in
+// `skip` get no cells.
+void mur_segments(int nx, int ny, int nz, int *seg_start, unsigned skip = 0);
 
 // ---- batches of independent grids -------------------------------------
 //
@@ -150,6 +178,8 @@ struct BatchInst {
     unsigned guard;
     float ch[3], cd[3];
     int kchunk, tiles_x, tiles_y, tiles_z;
+    int mur_x;                     // see StepArgs
+    const float4 *murx0, *murx1;   // see StepArgs
     int nprobe, probe_rows;        // probe_out holds probe_rows rows
     float dt, hx, hy, hz, mu0, eps0;
     int mur_seg[13];

@@ -71,6 +71,7 @@ __device__ inline float e_update(float c, float d, float a, double g1, double g2
 template <typename T> struct Elem;
 
 template <> struct Elem<float> {
+    static __device__ __forceinline__ float rnd(float v) { return v; }
     static __device__ __forceinline__ float ld(const float *p, uint32_t b) { return *(const float *)((const char *)p + b); }
     static __device__ __forceinline__ float ld_nt(const float *p, uint32_t b)
     {
@@ -86,6 +87,7 @@ template <> struct Elem<float> {
 template <> struct Elem<__half> {
     static __device__ __forceinline__ float cvt(unsigned short u) { return __half2float(__ushort_as_half(u)); }
     static __device__ __forceinline__ unsigned short enc(float v) { return __half_as_ushort(__float2half_rn(v)); }
+    static __device__ __forceinline__ float rnd(float v) { return cvt(enc(v)); }
     static __device__ __forceinline__ float ld(const __half *p, uint32_t b)
     {
         return cvt(*(const unsigned short *)((const char *)p + b));
@@ -110,6 +112,7 @@ template <> struct Elem<__hip_bfloat16> {
         if ((x & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((x >> 16) | 0x40u);
         return (unsigned short)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
     }
+    static __device__ __forceinline__ float rnd(float v) { return cvt(enc(v)); }
     static __device__ __forceinline__ float ld(const __hip_bfloat16 *p, uint32_t b)
     {
         return cvt(*(const unsigned short *)((const char *)p + b));
@@ -135,6 +138,17 @@ __device__ inline float e_update_scaled(float k1, float k2, float k3, float k5, 
     const float d1 = dn1 - dn;
     const float d2 = d1 - (dn - dm1);
     return k1 * e0 - k2 * e1 + k3 * d1 + k5 * d2;
+}
+
+// First-order Mur update of one face cell (fdtd_calculations.cu:237-427):
+// rpf/rpn are the face and neighbour permittivities, tn the neighbour at
+// n+1, sn and sf the neighbour and the face at n.
+__device__ inline float mur1(float dt, float h, float mu0, float eps0, float rpf, float rpn, float tn, float sn,
+                             float sf)
+{
+    return 1 / (dt + h * sqrtf(mu0 * eps0 * rpf)) *
+           ((dt - h * sqrtf(mu0 * eps0 * rpn)) * tn + (dt + h * sqrtf(mu0 * eps0 * rpn)) * sn -
+            (dt - h * sqrtf(mu0 * eps0 * rpf)) * sf);
 }
 
 // All per-lane addresses are a 64-bit array base (uniform, in SGPRs) plus a
@@ -402,11 +416,53 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
             eyn = e_update_scaled(k1, k2, k3, k5, eyk, cu.eym, dyp, cu.dyn, cu.dym);
             ezn = e_update_scaled(k1, k2, k3, k5, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
         }
+        // Mur x faces (fdtd_calculations.cu:303-329, 369-395) in the tile:
+        // E^{n+1} of the neighbour comes from the adjacent lane as it will be
+        // stored, E^n of the face and the neighbour from LDS.  Ez at k = 0 is
+        // outside the E update, so its neighbour is read from the target
+        // slot (not written in this step).
+        bool sty = reyz, stz = reyz;
+        if (A.mur_x) {
+            const float eyr = X::rnd(eyn), ezr = X::rnd(ezn);
+            if (i0 == 0) {
+                const float ey1 = __shfl_down(eyr, 1, 64), ez1 = __shfl_down(ezr, 1, 64);
+                if (tx == 0 && j < ny) {
+                    const float4 c = A.murx0[(long long)k * ny + j];
+                    if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                        eyn = c.x * ((c.y * ey1 + c.z * sE[cur][1][ty + 1][2]) - c.w * eyk);
+                        sty = true;
+                    }
+                    if (j >= 1 && j < ny - 1 && k < nz - 1) {
+                        const float tn = k >= 1 ? ez1 : X::ld(ezT, ob + ES);
+                        ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][2]) - c.w * ezk);
+                        stz = true;
+                    }
+                }
+            }
+            if (i0 <= nx - 1 && nx - 1 < i0 + TX) {
+                const float ezm = __shfl_up(ezr, 1, 64);
+                if (i == nx - 1 && j < ny) {
+                    const float4 c = A.murx1[(long long)k * ny + j];
+                    if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                        // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
+                        const float sn = sE[cur][1][ty + 1][tx];
+                        eyn = c.x * ((c.y * sn + c.z * sn) - c.w * eyk);
+                        sty = true;
+                    }
+                    if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
+                        const float tn = k >= 1 ? ezm : X::ld(ezT, ob - ES);
+                        ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][tx]) - c.w * ezk);
+                        stz = true;
+                    }
+                }
+            }
+        }
         if (own) {
             if (rdx) { X::st_nt(dxT, ob, dxu); X::st_nt(exT, ob, exn); }
             if (rdy) X::st_nt(dyT, ob, dyu);
             if (rdz) X::st_nt(dzT, ob, dzu);
-            if (reyz) { X::st_nt(eyT, ob, eyn); X::st_nt(ezT, ob, ezn); }
+            if (sty) X::st_nt(eyT, ob, eyn);
+            if (stz) X::st_nt(ezT, ob, ezn);
         }
         hxm = hx;
         hym = hy;
@@ -415,9 +471,11 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     }
 }
 
-// One grid, one step: block b runs tile xcd_logical(b).
+// One grid, one step: block b runs tile xcd_logical(b).  Five waves per
+// SIMD fit in 96 VGPRs without spilling (the persistent batch kernel would
+// spill, so it keeps four).
 template <int TY, typename T, bool EXACT>
-__global__ __launch_bounds__(TX *TY) void fused_step(const StepArgs A)
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(5))) void fused_step(const StepArgs A)
 {
     const int tid = threadIdx.y * TX + threadIdx.x;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
@@ -462,6 +520,9 @@ __device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, Step
     A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y; A.tiles_z = d.tiles_z;
     A.xcd_remap = 0;
     A.guard = d.guard;
+    A.mur_x = d.mur_x;
+    A.murx0 = d.murx0;
+    A.murx1 = d.murx1;
 }
 
 // Persistent fused kernel over every live instance of one storage class:
@@ -496,14 +557,6 @@ __global__ __launch_bounds__(TX *TY) void fused_batch(const BatchLaunch L)
         }
         fused_tile<TY, T, EXACT>(A, bx, by, bz);
     }
-}
-
-__device__ inline float mur1(float dt, float h, float mu0, float eps0, float rpf, float rpn, float tn, float sn,
-                             float sf)
-{
-    return 1 / (dt + h * sqrtf(mu0 * eps0 * rpf)) *
-           ((dt - h * sqrtf(mu0 * eps0 * rpn)) * tn + (dt + h * sqrtf(mu0 * eps0 * rpn)) * sn -
-            (dt - h * sqrtf(mu0 * eps0 * rpf)) * sf);
 }
 
 // Face cell t (in segment order) of the 12 (component, face) Mur updates;
@@ -729,7 +782,7 @@ __global__ void probe_cells(const ProbeArgs P)
 
 }  // namespace
 
-void mur_segments(int nx, int ny, int nz, int *seg)
+void mur_segments(int nx, int ny, int nz, int *seg, unsigned skip)
 {
     const int cnt[12] = {
         (nx - 1) * (nz - 2), (nx - 1) * (nz - 2), (nx - 1) * (ny - 2), (nx - 1) * (ny - 2),
@@ -737,7 +790,7 @@ void mur_segments(int nx, int ny, int nz, int *seg)
         (ny - 2) * (nz - 1), (ny - 2) * (nz - 1), (nx - 2) * (nz - 1), (nx - 2) * (nz - 1),
     };
     seg[0] = 0;
-    for (int q = 0; q < 12; q++) seg[q + 1] = seg[q] + std::max(cnt[q], 0);
+    for (int q = 0; q < 12; q++) seg[q + 1] = seg[q] + ((skip >> q & 1u) ? 0 : std::max(cnt[q], 0));
 }
 
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s)

@@ -190,7 +190,7 @@ def main():
 
 
 def _instance(F, w, storage="f32"):
-    ext = F.make_ext(storage={"f32": F.STORE_F32, "f16": F.STORE_F16}[storage])
+    ext = F.make_ext(storage={"f32": F.STORE_F32, "f16": F.STORE_F16, "bf16": F.STORE_BF16}[storage])
     return F.Instance(w.params, ext, w.ids, w.table, w.sources, w.jz, w.steps, w.probes)
 
 
@@ -237,7 +237,7 @@ def bench_batched(a):
     from fdtd_amd import workloads as W
 
     ws = W.batched_set()
-    insts = [_instance(F, w) for w in ws]
+    insts = [_instance(F, w, a.storage) for w in ws]
     work = sum(w.cells * w.steps for w in ws)
     cells = sum(w.cells for w in ws)
     arena = int(sum(x.nbytes() for x in insts) * 1.02) + (64 << 20)
@@ -266,7 +266,8 @@ def bench_batched(a):
         "metric": "Gcell-updates/s (64 uneven grids batched: one persistent fused launch per step)",
         "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 0,
         "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (f64 Debye sums)", "data": "synthetic (seeded phantoms)",
+        "dtype": "f32 (f64 Debye sums)" if a.storage == "f32" else f"{a.storage} storage, f32 arithmetic",
+        "data": "synthetic (seeded phantoms)",
         "config": {"workload": "batched_set", "grids": len(ws), "cells": cells, "cell_updates": work,
                    "steps_range": [min(w.steps for w in ws), steps], "tile_y": ty, "kchunk": kc,
                    "graph_steps": a.segment, "persistent_grid": info["grid"][0], "parallelism": "single GPU"},
@@ -276,6 +277,10 @@ def bench_batched(a):
                      "bytes_per_cell": 97},
         "upload_s": round(t_upload, 3), "all_finished": ok,
     }
+    if a.storage != "f32":
+        out["roofline"]["bytes_per_cell"] = BYTES_PER_CELL[a.storage]
+        out["roofline"]["achieved"] = round(BYTES_PER_CELL[a.storage] * work / (ms * 1e-3) / 1e9, 1)
+        out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBS, 4)
     if not a.no_sequential:
         tf = _sequential_fused(F, hip, insts)
         td = _sequential_dropin(F, insts)

@@ -253,6 +253,8 @@ int sync_tables(fdtd_batch *b, hipStream_t stream)
         auto &I = b->in[s];
         BatchInst &d = I.d;
         d.kchunk = std::min(b->kchunk, d.nz);
+        d.mur_x = fused_mur_x(d.nx, d.kchunk);
+        mur_segments(d.nx, d.ny, d.nz, d.mur_seg, mur_skip_mask(d.mur_x));
         d.tiles_x = (d.nx + 63) / 64;
         d.tiles_y = (d.ny + b->ty - 1) / b->ty;
         d.tiles_z = (d.nz + d.kchunk - 1) / d.kchunk;
@@ -514,10 +516,8 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     d.nprobe = in->nprobe;
     d.probe_rows = in->steps;
     d.dt = p->dt; d.hx = p->dx; d.hy = p->dy; d.hz = p->dz; d.mu0 = p->mu0; d.eps0 = p->eps0;
-    d.mur_x = fused_mur_x(nx);
     d.murx0 = (const float4 *)(base + L.murx);
     d.murx1 = d.murx0 + (size_t)ny * nz;
-    mur_segments(nx, ny, nz, d.mur_seg, mur_skip_mask(d.mur_x));
     I.start = b->step;
     I.used = true;
     b->dirty = true;

@@ -432,7 +432,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
     a.xcd_remap = c->ext.xcd_remap;
     a.guard = (unsigned)c->guard;
-    a.mur_x = fused_mur_x(c->nx);
+    a.mur_x = fused_mur_x(c->nx, c->kchunk);
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
     const bool timed = c->tused < c->tcap;

@@ -78,7 +78,14 @@ void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, cons
 // E^{n+1}_z at i = nx-2 from the neighbouring lane, which lives in the
 // previous tile when nx-1 is a multiple of 64; that face then stays in the
 // Mur kernel.
-inline int fused_mur_x(int nx) { return (nx - 1) % 64 == 0 ? 2 : 1; }
+// The fused kernel stages the coefficients of its z-chunk in LDS, so chunks
+// longer than kFusedMurMaxChunk planes leave the x faces to the Mur kernel.
+constexpr int kFusedMurMaxChunk = 16;
+inline int fused_mur_x(int nx, int kchunk)
+{
+    if (kchunk > kFusedMurMaxChunk) return 0;
+    return (nx - 1) % 64 == 0 ? 2 : 1;
+}
 // Mur segments (in mur_segments order) left to the Mur kernel.
 inline unsigned mur_skip_mask(int mur_x)
 {

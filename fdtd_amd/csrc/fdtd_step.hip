@@ -29,6 +29,7 @@
 namespace {
 
 constexpr int TX = 64;
+constexpr int kMurK = kFusedMurMaxChunk;  // z planes of staged Mur coefficients per tile
 
 __device__ inline int xcd_logical(int b, int nblk, int mode, int unit)
 {
@@ -236,10 +237,15 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     const uint8_t *ids = A.ids - A.guard;
 
     const uint32_t syb = (uint32_t)A.pitch * ES, szb = (uint32_t)A.plane * ES;
+    // Lanes past the row pitch (the last x tile of a row whose pitch is not a
+    // multiple of 64) read column pitch-1 instead: a line the lanes below
+    // them read anyway.  Their values are never used.  (A branch around the
+    // loads instead would make the compiler wait for each load inside it.)
+    const int il = min(i, A.pitch - 1);
     // plane-relative byte offsets (wrap-around arithmetic handles j0-1 = -1 etc.)
-    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)i * ES;
-    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)i * ES;
-    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)i * ES;
+    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)il * ES;
+    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)il * ES;
+    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)il * ES;
     const int side = tx >> 5;
     const int cr = min(tx & 31, TY + 1);             // LDS row of the gathered column cell
     const int cl = side ? TX + 1 : 0;                 // LDS column of the gathered cell
@@ -252,18 +258,12 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
 
     // E^n staging registers of one plane: own cell, -y row (wave 0), +y row
     // (wave TY-1, Ex and Ez), gathered columns (wave 1).
-    // Lanes past the row pitch (the last x tile of a row whose pitch is not a
-    // multiple of 64) load nothing: their values are never used, and their
-    // addresses would fall on the next row's first line.
-    const bool lane_in = i < A.pitch;
-    float pe[3] = {0.f, 0.f, 0.f}, pr[3] = {0.f, 0.f, 0.f}, pq[2] = {0.f, 0.f}, pc[3];
+    float pe[3], pr[3], pq[2], pc[3];
     auto fetch_e = [&](int k) {
         const uint32_t kb = (uint32_t)k * szb;
-        if (lane_in) {
-            pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
-            if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
-            if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
-        }
+        pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
+        if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
+        if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
         if (w_col) { pc[0] = X::ld(ex0, kb + c_col); pc[1] = X::ld(ey0, kb + c_col); pc[2] = X::ld(ez0, kb + c_col); }
     };
     auto stage_e = [&](int buf) {
@@ -280,22 +280,38 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     };
     auto fetch_own = [&](int k, Own &o) {
         const uint32_t ob = (uint32_t)k * szb + c_own;
+        o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
+        o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
+        o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
+        o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+        o.id = __builtin_nontemporal_load(ids + ob / ES);
+        // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TX, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;
-        if (lane_in) {
-            o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
-            o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
-            o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
-            o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
-            o.id = __builtin_nontemporal_load(ids + ob / ES);
-            // halo H^{n-1/2} of row (i, j0-1): Hx, Hz
-            if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
-        } else {
-            o.hxo = o.hyo = o.hzo = o.dxn = o.dyn = o.dzn = o.dxm = o.dym = o.dzm = o.exm = o.eym = o.ezm = 0.f;
-            o.id = 0;
-        }
-        // halo H^{n-1/2} of column (i0-1 | i0+TX, row cr): Hy, Hz
+        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
         if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
     };
+
+    // Mur x faces: the tile's coefficients (planes k0..k1-1, rows j0..j0+TY-1,
+    // both faces) and, on the k = 0 chunk, the never-updated Ez^{n+1}
+    // neighbours at k = 0 go to LDS up front, so the loop issues no global
+    // load beyond the streams.
+    __shared__ float4 sMurC[2][kMurK][TY];
+    __shared__ float sMurT[2][TY];
+    const bool mur_lo = A.mur_x && i0 == 0;
+    const bool mur_hi = A.mur_x && i0 <= nx - 1 && nx - 1 < i0 + TX;
+    if (mur_lo || mur_hi) {
+        const int tid = ty * TX + tx, nk = k1 - k0;
+        for (int q = tid; q < 2 * nk * TY; q += TX * TY) {
+            const int side = q / (nk * TY), r = q - side * nk * TY, kk = r / TY, row = r - kk * TY;
+            const long long at = (long long)(k0 + kk) * ny + min(j0 + row, ny - 1);
+            sMurC[side][kk][row] = side ? A.murx1[at] : A.murx0[at];
+        }
+        if (k0 == 0 && tid < 2 * TY) {
+            const int side = tid / TY, row = tid - side * TY;
+            const int ii = side ? nx - 2 : 1, jj = min(j0 + row, ny - 1);
+            sMurT[side][row] = X::ld(ezT, gb + (uint32_t)jj * syb + (uint32_t)ii * ES);
+        }
+    }
 
     float hxm = 0.0f, hym = 0.0f;  // own-column Hx, Hy of plane k-1 (time level n+1/2)
     int cur = 0;
@@ -307,11 +323,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         __syncthreads();
         const int k = k0 - 1;
         const uint32_t ob = (uint32_t)k * szb + c_own;
-        float hx0 = 0.f, hy0 = 0.f;
-        if (lane_in) {
-            hx0 = X::ld(hxo, ob);
-            hy0 = X::ld(hyo, ob);
-        }
+        const float hx0 = X::ld(hxo, ob), hy0 = X::ld(hyo, ob);
         const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
         const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
         const float ux = hx0 - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
@@ -422,38 +434,35 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         // outside the E update, so its neighbour is read from the target
         // slot (not written in this step).
         bool sty = reyz, stz = reyz;
-        if (A.mur_x) {
-            const float eyr = X::rnd(eyn), ezr = X::rnd(ezn);
-            if (i0 == 0) {
-                const float ey1 = __shfl_down(eyr, 1, 64), ez1 = __shfl_down(ezr, 1, 64);
-                if (tx == 0 && j < ny) {
-                    const float4 c = A.murx0[(long long)k * ny + j];
-                    if (j < ny - 1 && k >= 1 && k < nz - 1) {
-                        eyn = c.x * ((c.y * ey1 + c.z * sE[cur][1][ty + 1][2]) - c.w * eyk);
-                        sty = true;
-                    }
-                    if (j >= 1 && j < ny - 1 && k < nz - 1) {
-                        const float tn = k >= 1 ? ez1 : X::ld(ezT, ob + ES);
-                        ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][2]) - c.w * ezk);
-                        stz = true;
-                    }
+        if (mur_lo) {
+            const float ey1 = __shfl_down(X::rnd(eyn), 1, 64), ez1 = __shfl_down(X::rnd(ezn), 1, 64);
+            if (tx == 0 && j < ny) {
+                const float4 c = sMurC[0][k - k0][ty];
+                if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                    eyn = c.x * ((c.y * ey1 + c.z * sE[cur][1][ty + 1][2]) - c.w * eyk);
+                    sty = true;
+                }
+                if (j >= 1 && j < ny - 1 && k < nz - 1) {
+                    const float tn = k >= 1 ? ez1 : sMurT[0][ty];
+                    ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][2]) - c.w * ezk);
+                    stz = true;
                 }
             }
-            if (i0 <= nx - 1 && nx - 1 < i0 + TX) {
-                const float ezm = __shfl_up(ezr, 1, 64);
-                if (i == nx - 1 && j < ny) {
-                    const float4 c = A.murx1[(long long)k * ny + j];
-                    if (j < ny - 1 && k >= 1 && k < nz - 1) {
-                        // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
-                        const float sn = sE[cur][1][ty + 1][tx];
-                        eyn = c.x * ((c.y * sn + c.z * sn) - c.w * eyk);
-                        sty = true;
-                    }
-                    if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
-                        const float tn = k >= 1 ? ezm : X::ld(ezT, ob - ES);
-                        ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][tx]) - c.w * ezk);
-                        stz = true;
-                    }
+        }
+        if (mur_hi) {
+            const float ezm = __shfl_up(X::rnd(ezn), 1, 64);
+            if (i == nx - 1 && j < ny) {
+                const float4 c = sMurC[1][k - k0][ty];
+                if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                    // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
+                    const float sn = sE[cur][1][ty + 1][tx];
+                    eyn = c.x * ((c.y * sn + c.z * sn) - c.w * eyk);
+                    sty = true;
+                }
+                if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
+                    const float tn = k >= 1 ? ezm : sMurT[1][ty];
+                    ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][tx]) - c.w * ezk);
+                    stz = true;
                 }
             }
         }

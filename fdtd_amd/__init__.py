@@ -47,7 +47,7 @@ EXPORTS = {
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
-                   "fdtd_ctx_set_energy", "fdtd_energy_read",
+                   "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
@@ -167,6 +167,8 @@ def lib():
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
         L.fdtd_ctx_set_energy.argtypes = [vp, i, i]
+        L.fdtd_ctx_save.argtypes = [vp, C.c_char_p]
+        L.fdtd_ctx_load.argtypes = [vp, C.c_char_p]
         L.fdtd_energy_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         PP, PE, PI = C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(FdtdInstance)
@@ -290,6 +292,12 @@ class Context:
 
     def set_slices(self, k, interval, capacity):
         check(lib().fdtd_ctx_set_slices(self._h, k, interval, capacity), "set_slices")
+
+    def save(self, path):
+        check(lib().fdtd_ctx_save(self._h, str(path).encode()), "save")
+
+    def load(self, path):
+        check(lib().fdtd_ctx_load(self._h, str(path).encode()), "load")
 
     def set_energy(self, interval, capacity):
         check(lib().fdtd_ctx_set_energy(self._h, interval, capacity), "set_energy")

@@ -628,3 +628,84 @@ extern "C" int fdtd_energy_read(fdtd_ctx *c, int *steps, double *energy, int max
     for (int r = 0; r < rows && steps; r++) steps[r] = c->energy_steps[r];
     return rows;
 }
+
+// Binary snapshot: a fixed header, then the 24 field arrays in the device
+// layout and storage type (E slots, D slots, H buffers; pitched rows),
+// written as they are so a restore is bit-exact.  Materials, sources, rp
+// planes, probes and slices are inputs, not state: a restoring caller sets
+// them as before.
+namespace {
+
+struct SnapHeader {
+    char magic[8];            // "FDTDSNP1"
+    int nx, ny, nz, pitch;
+    long long plane;
+    int storage, esz, n, hcur;
+    int debye_mode, jz_index_mode;
+    double sE, sD, sH;
+    int scales_set, pad;
+};
+
+}  // namespace
+
+extern "C" int fdtd_ctx_save(fdtd_ctx *c, const char *path)
+{
+    if (!c || !path) return fdtd_fail(FDTD_EINVAL, "null argument");
+    resolve_scales(c);
+    HIP_TRY(hipDeviceSynchronize());
+    SnapHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, "FDTDSNP1", 8);
+    h.nx = c->nx; h.ny = c->ny; h.nz = c->nz; h.pitch = c->pitch; h.plane = c->plane;
+    h.storage = c->storage; h.esz = (int)c->esz; h.n = c->n; h.hcur = c->hcur;
+    h.debye_mode = c->ext.debye_mode; h.jz_index_mode = c->ext.jz_index_mode;
+    h.sE = c->sE; h.sD = c->sD; h.sH = c->sH; h.scales_set = c->scales_set ? 1 : 0;
+    FILE *f = fopen(path, "wb");
+    if (!f) return fdtd_fail(FDTD_EIO, "cannot write %s", path);
+    bool ok = fwrite(&h, sizeof h, 1, f) == 1;
+    std::vector<char> buf(c->Np * c->esz);
+    char *const *arrays[3] = {&c->E[0][0], &c->D[0][0], &c->H[0][0]};
+    const int counts[3] = {9, 9, 6};
+    for (int g = 0; g < 3 && ok; g++)
+        for (int q = 0; q < counts[g] && ok; q++) {
+            if (hipMemcpy(buf.data(), arrays[g][q], buf.size(), hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+            else ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+        }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : fdtd_fail(FDTD_EIO, "writing snapshot %s failed", path);
+}
+
+extern "C" int fdtd_ctx_load(fdtd_ctx *c, const char *path)
+{
+    if (!c || !path) return fdtd_fail(FDTD_EINVAL, "null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fdtd_fail(FDTD_EIO, "cannot read %s", path);
+    SnapHeader h;
+    if (fread(&h, sizeof h, 1, f) != 1 || memcmp(h.magic, "FDTDSNP1", 8) != 0) {
+        fclose(f);
+        return fdtd_fail(FDTD_EIO, "%s is not a snapshot", path);
+    }
+    if (h.nx != c->nx || h.ny != c->ny || h.nz != c->nz || h.pitch != c->pitch || h.plane != c->plane ||
+        h.storage != c->storage || h.esz != (int)c->esz) {
+        fclose(f);
+        return fdtd_fail(FDTD_ERANGE, "snapshot %s is of another grid or storage", path);
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<char> buf(c->Np * c->esz);
+    char *const *arrays[3] = {&c->E[0][0], &c->D[0][0], &c->H[0][0]};
+    const int counts[3] = {9, 9, 6};
+    bool ok = true;
+    for (int g = 0; g < 3 && ok; g++)
+        for (int q = 0; q < counts[g] && ok; q++) {
+            ok = fread(buf.data(), 1, buf.size(), f) == buf.size();
+            if (ok && hipMemcpy(arrays[g][q], buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) ok = false;
+        }
+    fclose(f);
+    if (!ok) return fdtd_fail(FDTD_EIO, "reading snapshot %s failed", path);
+    c->n = h.n;
+    c->hcur = h.hcur;
+    c->sE = h.sE; c->sD = h.sD; c->sH = h.sH;
+    c->scales_set = h.scales_set != 0;
+    c->lut_dirty = true;
+    return 0;
+}

@@ -126,6 +126,14 @@ int fdtd_ctx_layout(fdtd_ctx *ctx, int *pitch, long long *plane, int *elem_bytes
  * first step. */
 int fdtd_ctx_scales(fdtd_ctx *ctx, double *scale_e, double *scale_d, double *scale_h);
 
+/* Checkpoint and restart: write every field level, the step count and the
+ * storage scales to a binary file, bit for bit; load restores them into a
+ * context of the same grid and storage.  Materials, sources, rp planes,
+ * probes and slices are inputs and are set on the restoring context as on
+ * the original. */
+int fdtd_ctx_save(fdtd_ctx *ctx, const char *path);
+int fdtd_ctx_load(fdtd_ctx *ctx, const char *path);
+
 /* One full time step (H, D, sources, Debye E, Mur, probes) on `stream`. */
 int fdtd_step_fused(fdtd_ctx *ctx, hipStream_t stream);
 /* `steps` steps; enqueues only (no host synchronisation). */

@@ -56,3 +56,35 @@ def test_energy_decays_after_the_pulse_leaves():
     _, e = ctx.energy()
     assert np.isfinite(e).all()
     assert e[-1] < 0.05 * e.max()
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F32, F.STORE_F16])
+def test_snapshot_restart_is_bit_exact(tmp_path, storage):
+    """Run 12 steps, save, run 9 more; a fresh context with the same inputs
+    that loads the snapshot and runs 9 steps ends bit-equal."""
+    shape = (37, 21, 18)
+    rng = np.random.default_rng(5)
+    ids, table = random_materials(rng, shape, 4)
+    src = random_sources(rng, shape, 2)
+    jz = _jz()
+
+    def make():
+        ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
+        ctx.set_materials(ids, table)
+        ctx.set_sources(src, jz)
+        return ctx
+
+    a = make()
+    a.run(12)
+    a.save(tmp_path / "state.bin")
+    a.run(9)
+    b = make()
+    b.load(tmp_path / "state.bin")
+    assert b.n == 12
+    b.run(9)
+    for f in range(9):
+        for lv in ([0] if 3 <= f < 6 else [0, 1, 2]):
+            assert np.array_equal(a.get(f, lv).view(np.uint32), b.get(f, lv).view(np.uint32)), (f, lv)
+    other = F.Context(F.make_params((36, 21, 18), DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
+    with pytest.raises(F.FdtdError, match="-2"):
+        other.load(tmp_path / "state.bin")

@@ -186,10 +186,10 @@ struct fdtd_batch {
     int ty = 4, kchunk = 8, graph_steps = 8;
     bool dirty = true;                 // device tables out of date
     std::vector<int> running;          // slots in the device tables
-    int mask = 0;                      // storage forms in the device tables
-    int grid[kBatchClasses] = {0, 0, 0, 0};
-    int tiles[kBatchClasses] = {0, 0, 0, 0};
-    long long cells[kBatchClasses] = {0, 0, 0, 0};
+    int mask = 0;                      // kernel classes in the device tables
+    int grid[kBatchClasses] = {};
+    int tiles[kBatchClasses] = {};
+    long long cells[kBatchClasses] = {};
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     hipStream_t last = nullptr;
     bool ran = false;
@@ -253,14 +253,15 @@ int sync_tables(fdtd_batch *b, hipStream_t stream)
         auto &I = b->in[s];
         BatchInst &d = I.d;
         d.kchunk = std::min(b->kchunk, d.nz);
-        d.mur_x = fused_mur_x(d.nx, d.kchunk);
+        const int tw = 64 * d.cpl;
+        d.mur_x = fused_mur_x(d.nx, d.kchunk, tw);
         mur_segments(d.nx, d.ny, d.nz, d.mur_seg, mur_skip_mask(d.mur_x));
-        d.tiles_x = (d.nx + 63) / 64;
+        d.tiles_x = (d.nx + tw - 1) / tw;
         d.tiles_y = (d.ny + b->ty - 1) / b->ty;
         d.tiles_z = (d.nz + d.kchunk - 1) / d.kchunk;
         T->inst[s] = d;
         maxslot = std::max(maxslot, s);
-        const int c = d.storage;
+        const int c = batch_class(d.storage, d.cpl);
         mask |= 1 << c;
         BatchClass &C = T->cls[c];
         const int nt = d.tiles_x * d.tiles_y * d.tiles_z;
@@ -509,6 +510,7 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     d.start = (int)b->step;
     d.nsteps = in->steps;
     d.storage = e.storage;
+    d.cpl = auto_lane_cells(nx, L.esz, e.lane_cells);
     d.nx = nx; d.ny = ny; d.nz = nz; d.pitch = L.pitch;
     d.guard = (unsigned)L.guard;
     const GridGeom g{p->dt, p->dx, p->dy, p->dz, p->mu0, p->eps0};
@@ -725,10 +727,11 @@ extern "C" int fdtd_batch_info(fdtd_batch *b, FdtdBatchInfo *info)
     info->tile_y = b->ty;
     info->kchunk = b->kchunk;
     info->graph_steps = b->graph_steps;
-    for (int c = 0; c < kBatchClasses; c++) {
-        info->grid[c] = fused_grid(b, c);
-        info->tiles[c] = b->tiles[c];
-        info->cells[c] = b->cells[c];
+    for (int c = 0; c < kBatchClasses; c++) {  // per storage form
+        const int st = class_storage(c);
+        info->grid[st] = std::max(info->grid[st], fused_grid(b, c));
+        info->tiles[st] += b->tiles[c];
+        info->cells[st] += b->cells[c];
     }
     return 0;
 }

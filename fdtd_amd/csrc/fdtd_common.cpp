@@ -241,6 +241,7 @@ int fdtd_check_ext(const FdtdExt &e)
         e.storage != FDTD_STORE_F32_FAST)
         return fdtd_fail(FDTD_EINVAL, "storage %d", e.storage);
     if (!(e.scale_e >= 0.0f && e.scale_d >= 0.0f && e.scale_h >= 0.0f)) return fdtd_fail(FDTD_EINVAL, "negative scale");
+    if (e.lane_cells < 0 || e.lane_cells > 2) return fdtd_fail(FDTD_EINVAL, "lane_cells %d", e.lane_cells);
     return 0;
 }
 

@@ -38,6 +38,7 @@ struct fdtd_ctx {
     int storage;
     size_t esz;                          // bytes per stored element
     int ty, kchunk, tiles_x, tiles_y, tiles_z;
+    int cpl = 1;                         // cells per lane of the fused kernel
 
     long long guard = 0;                 // elements before/after the field arrays and the ids
     char *blob = nullptr;                // all field arrays
@@ -152,6 +153,7 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->xcd_remap = 0;  // fastest measured; 2 cuts fabric re-reads to 1.10x but runs ~5% slower
     ext->storage = FDTD_STORE_F32;
     ext->scale_e = ext->scale_d = ext->scale_h = 0.0f;
+    ext->lane_cells = 0;
     return 0;
 }
 
@@ -187,7 +189,9 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     // orders): 64x4 tiles marching 4 planes, identity order, within 1-3% of
     // every other shape at both sizes.
     c->ty = e.tile_y ? e.tile_y : 4;
-    c->tiles_x = (c->nx + 63) / 64;
+    // 16-bit fields: two cells per lane (one 32-bit access per stream) where the width suits 128-cell tiles
+    c->cpl = auto_lane_cells(c->nx, c->esz, e.lane_cells);
+    c->tiles_x = (c->nx + 64 * c->cpl - 1) / (64 * c->cpl);
     c->tiles_y = (c->ny + c->ty - 1) / c->ty;
     c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : std::min(4, c->nz);
     c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
@@ -432,7 +436,8 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.tiles_x = c->tiles_x; a.tiles_y = c->tiles_y; a.tiles_z = c->tiles_z;
     a.xcd_remap = c->ext.xcd_remap;
     a.guard = (unsigned)c->guard;
-    a.mur_x = fused_mur_x(c->nx, c->kchunk);
+    a.mur_x = fused_mur_x(c->nx, c->kchunk, 64 * c->cpl);
+    a.cpl = c->cpl;
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
     const bool timed = c->tused < c->tcap;
@@ -548,7 +553,7 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
 extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int *kchunk, int *blocks)
 {
     if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
-    if (tile_x) *tile_x = 64;
+    if (tile_x) *tile_x = 64 * c->cpl;
     if (tile_y) *tile_y = c->ty;
     if (kchunk) *kchunk = c->kchunk;
     if (blocks) *blocks = c->tiles_x * c->tiles_y * c->tiles_z;

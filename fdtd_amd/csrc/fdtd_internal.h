@@ -62,6 +62,7 @@ struct StepArgs {
     // all four (Ey, Ez at i = 0 and nx-1), 2 all but Ez at i = nx-1.
     int mur_x;
     const float4 *murx0, *murx1;    // per (k, j) coefficients of the x-low / x-high faces (mur_x_coefs)
+    int cpl;                        // cells per lane: 1, or 2 for 16-bit storage (128-cell tiles)
 };
 
 // Coefficients {q, A, B, C} of the first-order Mur update of one x-face
@@ -75,16 +76,16 @@ void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, cons
                  float4 *out0, float4 *out1);
 
 // Which Mur x faces the fused kernel takes over: the x-high Ez face needs
-// E^{n+1}_z at i = nx-2 from the neighbouring lane, which lives in the
-// previous tile when nx-1 is a multiple of 64; that face then stays in the
-// Mur kernel.
+// E^{n+1}_z at i = nx-2, which lives in the previous tile when nx-1 is a
+// multiple of the tile width (64 cells, or 128 with two cells per lane);
+// that face then stays in the Mur kernel.
 // The fused kernel stages the coefficients of its z-chunk in LDS, so chunks
 // longer than kFusedMurMaxChunk planes leave the x faces to the Mur kernel.
 constexpr int kFusedMurMaxChunk = 16;
-inline int fused_mur_x(int nx, int kchunk)
+inline int fused_mur_x(int nx, int kchunk, int tile_cells)
 {
     if (kchunk > kFusedMurMaxChunk) return 0;
-    return (nx - 1) % 64 == 0 ? 2 : 1;
+    return (nx - 1) % tile_cells == 0 ? 2 : 1;
 }
 // Mur segments (in mur_segments order) left to the Mur kernel.
 inline unsigned mur_skip_mask(int mur_x)
@@ -162,7 +163,26 @@ void mur_segments(int nx, int ny, int nz, int *seg_start, unsigned skip = 0);
 
 constexpr int kBatchMaxInst = 512;        // descriptors per batch
 constexpr int kBatchMaxProbes = 64;       // probes per instance
-constexpr int kBatchClasses = 4;          // one per Storage value
+// Kernel classes of a batch: the four storage forms with one cell per lane
+// (class = Storage value), then fp16 and bf16 with two cells per lane.
+constexpr int kBatchClasses = 6;
+inline int batch_class(int storage, int cpl)
+{
+    return cpl == 2 ? (storage == kStoreF16 ? 4 : 5) : storage;
+}
+inline int class_storage(int cls) { return cls < 4 ? cls : (cls == 4 ? kStoreF16 : kStoreBF16); }
+
+// Cells per lane of the fused kernel for a grid nx wide: 16-bit storage runs
+// two per lane (128-cell tiles) unless that leaves clearly more lanes idle
+// than 64-cell tiles (a pair tile costs ~0.82 of a single one per cell, as
+// measured at 256^3); `requested` 1 or 2 overrides.
+inline int auto_lane_cells(int nx, size_t esz, int requested)
+{
+    if (esz != 2) return 1;
+    if (requested == 1 || requested == 2) return requested;
+    const double e1 = (double)nx / (64.0 * ((nx + 63) / 64)), e2 = (double)nx / (128.0 * ((nx + 127) / 128));
+    return e2 >= 0.85 * e1 ? 2 : 1;
+}
 constexpr int kMurBlock = 256;            // Mur face cells per block
 
 // Device descriptor of one grid of a batch.  Instance step n (0-based) runs
@@ -187,6 +207,7 @@ struct BatchInst {
     int kchunk, tiles_x, tiles_y, tiles_z;
     int mur_x;                     // see StepArgs
     const float4 *murx0, *murx1;   // see StepArgs
+    int cpl;                       // cells per lane (kernel class)
     int nprobe, probe_rows;        // probe_out holds probe_rows rows
     float dt, hx, hy, hz, mu0, eps0;
     int mur_seg[13];
@@ -220,12 +241,12 @@ struct BatchLaunch {
 
 // LDS bytes of the batch's fused kernel (coefficient table of kMaxMaterials).
 size_t batch_fused_lds();
-hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int storage, int grid, hipStream_t s);
+hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int cls, int grid, hipStream_t s);
 hipError_t launch_mur_batch(const BatchLaunch &l, int grid, hipStream_t s);
 hipError_t launch_probe_batch(const BatchLaunch &l, int grid, hipStream_t s);
 hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s);
-// Resident blocks of the fused batch kernel per CU.
-int batch_fused_occupancy(int ty, int storage);
+// Resident blocks of the fused batch kernel of a class per CU.
+int batch_fused_occupancy(int ty, int cls);
 
 // ---- shared host helpers (fdtd_common.cpp) ------------------------------
 

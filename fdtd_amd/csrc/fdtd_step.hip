@@ -480,6 +480,345 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     }
 }
 
+// 16-bit pairs: one 32-bit word holds the values of cells (2m, 2m+1) of a row.
+template <typename T> struct Pair;
+template <> struct Pair<__half> {
+    static __device__ __forceinline__ float lo(uint32_t w) { return Elem<__half>::cvt((unsigned short)(w & 0xffffu)); }
+    static __device__ __forceinline__ float hi(uint32_t w) { return Elem<__half>::cvt((unsigned short)(w >> 16)); }
+    static __device__ __forceinline__ uint32_t pack(float a, float b)
+    {
+        return (uint32_t)Elem<__half>::enc(a) | ((uint32_t)Elem<__half>::enc(b) << 16);
+    }
+};
+template <> struct Pair<__hip_bfloat16> {
+    static __device__ __forceinline__ float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    static __device__ __forceinline__ float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+    static __device__ __forceinline__ uint32_t pack(float a, float b)
+    {
+        return (uint32_t)Elem<__hip_bfloat16>::enc(a) | ((uint32_t)Elem<__hip_bfloat16>::enc(b) << 16);
+    }
+};
+
+__device__ __forceinline__ uint32_t ldw(const void *p, uint32_t b) { return *(const uint32_t *)((const char *)p + b); }
+__device__ __forceinline__ uint32_t ldw_nt(const void *p, uint32_t b)
+{
+    return __builtin_nontemporal_load((const uint32_t *)((const char *)p + b));
+}
+__device__ __forceinline__ void stw_nt(void *p, uint32_t b, uint32_t w)
+{
+    __builtin_nontemporal_store(w, (uint32_t *)((char *)p + b));
+}
+
+// The fused step of one 128 x TY x kchunk tile for 16-bit storage, two
+// cells per lane: lane l of a wave holds cells (i0 + 2l, i0 + 2l + 1) of its
+// row, so every stream moves one 32-bit word per lane (256 bytes per wave
+// access) instead of 16 bits.  Each cell's arithmetic is that of
+// fused_tile<TY, T, false>, operation for operation, so the two kernels give
+// identical results.  Pairs are split into 16-bit stores only where the two
+// cells' update ranges differ (the x boundaries and the padding).
+template <int TY, typename T>
+__device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int by, int bz)
+{
+    static_assert(sizeof(T) == 2, "pairs of 16-bit cells");
+    static_assert(TY >= 2 && TY <= 8, "tile rows");
+    constexpr int TC = 2 * TX;  // cells per tile row
+    constexpr uint32_t ES = 2;
+    using X = Elem<T>;
+    using P = Pair<T>;
+    __shared__ float sE[2][3][TY + 2][TC + 2];
+    __shared__ float sH[3][TY + 1][TC + 1];
+    __shared__ float sJunk[2][TX];
+    __shared__ float4 sMurC[2][kMurK][TY];
+    __shared__ float sMurT[2][TY];
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int i0 = bx * TC, j0 = by * TY;
+    const int k0 = bz * A.kchunk;
+    const int k1 = min(k0 + A.kchunk, nz);
+    const int ia = i0 + 2 * tx;  // cells ia (a) and ia + 1 (b)
+    const int j = j0 + ty;
+    const int ca = 2 * tx + 1;   // LDS column of cell a; b is ca + 1
+    const bool own_a = ia < nx && j < ny, own_b = ia + 1 < nx && j < ny;
+
+    const int nmat = A.nmat;
+    float *lK1 = (float *)lut_lds(), *lK2 = lK1 + nmat, *lK3 = lK1 + 2 * nmat, *lK5 = lK1 + 3 * nmat;
+
+    const uint32_t gb = A.guard * ES;
+    auto sh = [&](const void *p) { return (const T *)((const char *)p - gb); };
+    auto shw = [&](void *p) { return (T *)((char *)p - gb); };
+    const T *hxo = sh(A.hxo), *hyo = sh(A.hyo), *hzo = sh(A.hzo);
+    const T *ex0 = sh(A.ex0), *ey0 = sh(A.ey0), *ez0 = sh(A.ez0);
+    const T *ex1 = sh(A.ex1), *ey1 = sh(A.ey1), *ez1 = sh(A.ez1);
+    const T *dx0 = sh(A.dx0), *dy0 = sh(A.dy0), *dz0 = sh(A.dz0);
+    const T *dx1 = sh(A.dx1), *dy1 = sh(A.dy1), *dz1 = sh(A.dz1);
+    T *hxn = shw(A.hxn), *hyn = shw(A.hyn), *hzn = shw(A.hzn);
+    T *exT = shw(A.exT), *eyT = shw(A.eyT), *ezT = shw(A.ezT);
+    T *dxT = shw(A.dxT), *dyT = shw(A.dyT), *dzT = shw(A.dzT);
+    const uint8_t *ids = A.ids - A.guard;
+
+    const uint32_t syb = (uint32_t)A.pitch * ES, szb = (uint32_t)A.plane * ES;
+    // Pairs past the row pitch read the last pair of the row (unused values).
+    const int il = min(ia, A.pitch - 2);
+    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)il * ES;
+    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)il * ES;
+    const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)il * ES;
+    const int side = tx >> 5;
+    const int cr = min(tx & 31, TY + 1);
+    const int cl = side ? TC + 1 : 0;
+    const int ci = side ? i0 + TC : i0 - 1;
+    const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * ES;
+    const bool w_rlo = (ty == 0), w_rhi = (ty == TY - 1), w_col = (ty == 1);
+
+    const float chx = A.chx, chy = A.chy, chz = A.chz;
+    const float cdx = A.cdx, cdy = A.cdy, cdz = A.cdz;
+
+    // E^n staging: the lane's pair (word), -y row pair (wave 0), +y row pair
+    // of Ex and Ez (wave TY-1), gathered column cells (wave 1).
+    uint32_t pe[3], pr[3], pq[2];
+    float pc[3];
+    auto fetch_e = [&](int k) {
+        const uint32_t kb = (uint32_t)k * szb;
+        pe[0] = ldw(ex0, kb + c_own); pe[1] = ldw(ey0, kb + c_own); pe[2] = ldw(ez0, kb + c_own);
+        if (w_rlo) { pr[0] = ldw(ex0, kb + c_rlo); pr[1] = ldw(ey0, kb + c_rlo); pr[2] = ldw(ez0, kb + c_rlo); }
+        if (w_rhi) { pq[0] = ldw(ex0, kb + c_rhi); pq[1] = ldw(ez0, kb + c_rhi); }
+        if (w_col) { pc[0] = X::ld(ex0, kb + c_col); pc[1] = X::ld(ey0, kb + c_col); pc[2] = X::ld(ez0, kb + c_col); }
+    };
+    auto stage_e = [&](int buf) {
+        for (int c = 0; c < 3; c++) {
+            sE[buf][c][ty + 1][ca] = P::lo(pe[c]);
+            sE[buf][c][ty + 1][ca + 1] = P::hi(pe[c]);
+        }
+        if (w_rlo)
+            for (int c = 0; c < 3; c++) {
+                sE[buf][c][0][ca] = P::lo(pr[c]);
+                sE[buf][c][0][ca + 1] = P::hi(pr[c]);
+            }
+        if (w_rhi) {
+            sE[buf][0][TY + 1][ca] = P::lo(pq[0]); sE[buf][0][TY + 1][ca + 1] = P::hi(pq[0]);
+            sE[buf][2][TY + 1][ca] = P::lo(pq[1]); sE[buf][2][TY + 1][ca + 1] = P::hi(pq[1]);
+        }
+        if (w_col) { sE[buf][0][cr][cl] = pc[0]; sE[buf][1][cr][cl] = pc[1]; sE[buf][2][cr][cl] = pc[2]; }
+    };
+
+    // Own-pair operands of one plane (words), two halo words, the id pair.
+    struct Own {
+        uint32_t hxo, hyo, hzo, dxn, dyn, dzn, dxm, dym, dzm, exm, eym, ezm, hh0, hh1;
+        uint32_t id;
+    };
+    auto fetch_own = [&](int k, Own &o) {
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        o.hxo = ldw(hxo, ob); o.hyo = ldw(hyo, ob); o.hzo = ldw(hzo, ob);
+        o.dxn = ldw_nt(dx0, ob); o.dyn = ldw_nt(dy0, ob); o.dzn = ldw_nt(dz0, ob);
+        o.dxm = ldw_nt(dx1, ob); o.dym = ldw_nt(dy1, ob); o.dzm = ldw_nt(dz1, ob);
+        o.exm = ldw_nt(ex1, ob); o.eym = ldw_nt(ey1, ob); o.ezm = ldw_nt(ez1, ob);
+        o.id = __builtin_nontemporal_load((const unsigned short *)(ids + ob / ES));
+        // halo H^{n-1/2}: row pair (ia, j0-1): Hx, Hz words; column cell (i0-1 | i0+TC, row cr): Hy, Hz
+        o.hh0 = 0u; o.hh1 = 0u;
+        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = ldw(hxo, oh); o.hh1 = ldw(hzo, oh); }
+        if (w_col) {
+            const uint32_t oh = (uint32_t)k * szb + c_col;
+            o.hh0 = __float_as_uint(X::ld(hyo, oh));
+            o.hh1 = __float_as_uint(X::ld(hzo, oh));
+        }
+    };
+    auto half_of = [](uint32_t w, int h) { return h ? P::hi(w) : P::lo(w); };
+
+    // Mur x faces, staged as in fused_tile.
+    const bool mur_lo = A.mur_x && i0 == 0;
+    const bool mur_hi = A.mur_x && i0 <= nx - 1 && nx - 1 < i0 + TC;
+    if (mur_lo || mur_hi) {
+        const int tid = ty * TX + tx, nk = k1 - k0;
+        for (int q = tid; q < 2 * nk * TY; q += TX * TY) {
+            const int sd = q / (nk * TY), r = q - sd * nk * TY, kk = r / TY, row = r - kk * TY;
+            const long long at = (long long)(k0 + kk) * ny + min(j0 + row, ny - 1);
+            sMurC[sd][kk][row] = sd ? A.murx1[at] : A.murx0[at];
+        }
+        if (k0 == 0 && tid < 2 * TY) {
+            const int sd = tid / TY, row = tid - sd * TY;
+            const int ii = sd ? nx - 2 : 1, jj = min(j0 + row, ny - 1);
+            sMurT[sd][row] = X::ld(ezT, gb + (uint32_t)jj * syb + (uint32_t)ii * ES);
+        }
+    }
+
+    float hxm[2] = {0.f, 0.f}, hym[2] = {0.f, 0.f};  // own-column Hx, Hy of plane k-1
+    int cur = 0;
+    if (k0 > 0) {
+        fetch_e(k0 - 1);
+        stage_e(0);
+        fetch_e(k0);
+        stage_e(1);
+        __syncthreads();
+        const int k = k0 - 1;
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        const uint32_t hx0 = ldw(hxo, ob), hy0 = ldw(hyo, ob);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = ca + h, i = ia + h;
+            const float hxv = half_of(hx0, h), hyv = half_of(hy0, h);
+            const float exk = sE[0][0][ty + 1][c], eyk = sE[0][1][ty + 1][c], ezk = sE[0][2][ty + 1][c];
+            const float exk1 = sE[1][0][ty + 1][c], eyk1 = sE[1][1][ty + 1][c];
+            const float ux = hxv - chy * (sE[0][2][ty + 2][c] - ezk) + chy * (eyk1 - eyk);
+            const float uy = hyv - chz * (exk1 - exk) + chx * (sE[0][2][ty + 1][c + 1] - ezk);
+            hxm[h] = (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1) ? ux : hxv;
+            hym[h] = (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1) ? uy : hyv;
+        }
+        cur = 1;
+        __syncthreads();
+    } else {
+        fetch_e(k0);
+        stage_e(0);
+    }
+    Own cu, nx_;
+    fetch_own(k0, cu);
+    fetch_e(k0 + 1);
+
+    const int jh = j0 - 1;
+    const int ccr = min(max(cr, 1), TY);
+    const int jc = j0 - 1 + ccr;
+    const bool col_ok = (side == 0) && (cr >= 1) && (cr <= TY) && (i0 >= 1);
+    const bool cy_ij = col_ok && (i0 - 1 < nx - 1) && (jc >= 1 && jc < ny - 1);
+    const bool cz_ij = col_ok && (i0 - 1 < nx - 1) && (jc < ny - 1);
+    float *col_dst_y = col_ok ? &sH[1][ccr][0] : &sJunk[0][tx];
+    float *col_dst_z = col_ok ? &sH[2][ccr][0] : &sJunk[1][tx];
+
+    const float jzv = A.jz;
+    for (int k = k0; k < k1; ++k) {
+        const int nxt = cur ^ 1;
+        stage_e(nxt);
+        const bool more = (k + 1 < k1);
+        if (more) {
+            fetch_own(k + 1, nx_);
+            fetch_e(k + 2);
+        }
+        __syncthreads();
+
+        const bool kh = (k < nz - 1), kz = (k >= 1 && k < nz - 1);
+        float hx[2], hy[2], hz[2], exk[2], eyk[2], ezk[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = ca + h, i = ia + h;
+            exk[h] = sE[cur][0][ty + 1][c]; eyk[h] = sE[cur][1][ty + 1][c]; ezk[h] = sE[cur][2][ty + 1][c];
+            const float exk1 = sE[nxt][0][ty + 1][c], eyk1 = sE[nxt][1][ty + 1][c];
+            const float hxo_ = half_of(cu.hxo, h), hyo_ = half_of(cu.hyo, h), hzo_ = half_of(cu.hzo, h);
+            const float ux = hxo_ - chy * (sE[cur][2][ty + 2][c] - ezk[h]) + chy * (eyk1 - eyk[h]);
+            const float uy = hyo_ - chz * (exk1 - exk[h]) + chx * (sE[cur][2][ty + 1][c + 1] - ezk[h]);
+            const float uz = hzo_ - chx * (sE[cur][1][ty + 1][c + 1] - eyk[h]) + chy * (sE[cur][0][ty + 2][c] - exk[h]);
+            hx[h] = (i >= 1 && i < nx - 1 && j < ny - 1 && kh) ? ux : hxo_;
+            hy[h] = (i < nx - 1 && j >= 1 && j < ny - 1 && kh) ? uy : hyo_;
+            hz[h] = (i < nx - 1 && j < ny - 1 && kz) ? uz : hzo_;
+            sH[0][ty + 1][c] = hx[h];
+            sH[1][ty + 1][c] = hy[h];
+            sH[2][ty + 1][c] = hz[h];
+        }
+        if (w_rlo) {  // row pair (ia + h, j0-1): Hx, Hz
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = ca + h, i = ia + h;
+                const float h0 = half_of(cu.hh0, h), h1 = half_of(cu.hh1, h);
+                const float ezh = sE[cur][2][0][c], eyh = sE[cur][1][0][c], exh = sE[cur][0][0][c];
+                const float vx = h0 - chy * (sE[cur][2][1][c] - ezh) + chy * (sE[nxt][1][0][c] - eyh);
+                const float vz = h1 - chx * (sE[cur][1][0][c + 1] - eyh) + chy * (sE[cur][0][1][c] - exh);
+                const bool rx = (jh >= 0) && (i >= 1 && i < nx - 1 && jh < ny - 1);
+                const bool rz = (jh >= 0) && (i < nx - 1 && jh < ny - 1);
+                sH[0][0][c] = (rx && kh) ? vx : h0;
+                sH[2][0][c] = (rz && kz) ? vz : h1;
+            }
+        }
+        if (w_col) {  // (i0-1, j0-1+ccr): Hy, Hz
+            const float h0 = __uint_as_float(cu.hh0), h1 = __uint_as_float(cu.hh1);
+            const float exh = sE[cur][0][ccr][0], eyh = sE[cur][1][ccr][0], ezh = sE[cur][2][ccr][0];
+            const float vy = h0 - chz * (sE[nxt][0][ccr][0] - exh) + chx * (sE[cur][2][ccr][1] - ezh);
+            const float vz = h1 - chx * (sE[cur][1][ccr][1] - eyh) + chy * (sE[cur][0][ccr + 1][0] - exh);
+            *col_dst_y = (cy_ij && kh) ? vy : h0;
+            *col_dst_z = (cz_ij && kz) ? vz : h1;
+        }
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        // one store of a pair, or of its first cell alone at the row's end
+        auto st_pair = [&](T *p, bool sa, bool sb, float va, float vb) {
+            if (sa && sb) stw_nt(p, ob, P::pack(va, vb));
+            else if (sa) X::st_nt(p, ob, va);
+            else if (sb) X::st_nt(p, ob + ES, vb);
+        };
+        st_pair(hxn, own_a, own_b, hx[0], hx[1]);
+        st_pair(hyn, own_a, own_b, hy[0], hy[1]);
+        st_pair(hzn, own_a, own_b, hz[0], hz[1]);
+        __syncthreads();
+
+        float dxu[2], dyu[2], dzu[2], exn[2], eyn[2], ezn[2];
+        bool rdx[2], rdy[2], rdz[2], reyz[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = ca + h, i = ia + h;
+            const bool own = h ? own_b : own_a;
+            rdx[h] = (i < nx - 1 && j >= 1 && j < ny - 1) && kz;
+            rdy[h] = (i >= 1 && i < nx - 1 && j < ny - 1) && kz;
+            rdz[h] = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) && kh;
+            reyz[h] = (i >= 1) && (k >= 1);
+            const float dxn_ = half_of(cu.dxn, h), dyn_ = half_of(cu.dyn, h), dzn_ = half_of(cu.dzn, h);
+            dxu[h] = dxn_ + cdy * (hz[h] - sH[2][ty][c]) - cdz * (hy[h] - hym[h]);
+            dyu[h] = dyn_ + cdz * (hx[h] - hxm[h]) - cdx * (hz[h] - sH[2][ty + 1][c - 1]);
+            float dz_ = dzn_ + cdx * (hy[h] - sH[1][ty + 1][c - 1]) - cdy * (hx[h] - sH[0][ty][c]);
+            const uint32_t id = (cu.id >> (8 * h)) & 0xffu;
+            dz_ = (id & kSourceBit) ? dz_ - jzv : dz_;
+            dzu[h] = dz_;
+            float dyp = dyu[h], dzp = dz_;
+            if (own && reyz[h] && !rdy[h]) dyp = X::ld(dyT, ob + h * ES);
+            if (own && reyz[h] && !rdz[h]) dzp = X::ld(dzT, ob + h * ES);
+            const int m = id & kMaterialMask;
+            const float k1c = lK1[m], k2c = lK2[m], k3c = lK3[m], k5c = lK5[m];
+            exn[h] = e_update_scaled(k1c, k2c, k3c, k5c, exk[h], half_of(cu.exm, h), dxu[h], dxn_, half_of(cu.dxm, h));
+            eyn[h] = e_update_scaled(k1c, k2c, k3c, k5c, eyk[h], half_of(cu.eym, h), dyp, dyn_, half_of(cu.dym, h));
+            ezn[h] = e_update_scaled(k1c, k2c, k3c, k5c, ezk[h], half_of(cu.ezm, h), dzp, dzn_, half_of(cu.dzm, h));
+        }
+        bool sty[2] = {reyz[0], reyz[1]}, stz[2] = {reyz[0], reyz[1]};
+        if (mur_lo && tx == 0 && j < ny) {  // face i = 0 is cell a of lane 0, its neighbour cell b
+            const float4 cm = sMurC[0][k - k0][ty];
+            if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                eyn[0] = cm.x * ((cm.y * X::rnd(eyn[1]) + cm.z * sE[cur][1][ty + 1][2]) - cm.w * eyk[0]);
+                sty[0] = true;
+            }
+            if (j >= 1 && j < ny - 1 && k < nz - 1) {
+                const float tn = k >= 1 ? X::rnd(ezn[1]) : sMurT[0][ty];
+                ezn[0] = cm.x * ((cm.y * tn + cm.z * sE[cur][2][ty + 1][2]) - cm.w * ezk[0]);
+                stz[0] = true;
+            }
+        }
+        if (mur_hi) {
+            // face i = nx-1 is cell b of the lane holding nx-2 (odd offset) or
+            // cell a with the neighbour in the previous lane's cell b
+            const bool face_b = ((nx - 1 - i0) & 1) != 0;
+            const float ezm = __shfl_up(X::rnd(ezn[1]), 1, 64);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (ia + h != nx - 1 || j >= ny || (h == 1) != face_b) continue;
+                const float4 cm = sMurC[1][k - k0][ty];
+                const int c = ca + h;
+                if (j < ny - 1 && k >= 1 && k < nz - 1) {
+                    const float sn = sE[cur][1][ty + 1][c - 1];
+                    eyn[h] = cm.x * ((cm.y * sn + cm.z * sn) - cm.w * eyk[h]);
+                    sty[h] = true;
+                }
+                if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
+                    const float tn = k >= 1 ? (h ? X::rnd(ezn[0]) : ezm) : sMurT[1][ty];
+                    ezn[h] = cm.x * ((cm.y * tn + cm.z * sE[cur][2][ty + 1][c - 1]) - cm.w * ezk[h]);
+                    stz[h] = true;
+                }
+            }
+        }
+        st_pair(dxT, own_a && rdx[0], own_b && rdx[1], dxu[0], dxu[1]);
+        st_pair(exT, own_a && rdx[0], own_b && rdx[1], exn[0], exn[1]);
+        st_pair(dyT, own_a && rdy[0], own_b && rdy[1], dyu[0], dyu[1]);
+        st_pair(dzT, own_a && rdz[0], own_b && rdz[1], dzu[0], dzu[1]);
+        st_pair(eyT, own_a && sty[0], own_b && sty[1], eyn[0], eyn[1]);
+        st_pair(ezT, own_a && stz[0], own_b && stz[1], ezn[0], ezn[1]);
+        hxm[0] = hx[0]; hxm[1] = hx[1];
+        hym[0] = hy[0]; hym[1] = hy[1];
+        cur = nxt;
+        if (more) cu = nx_;
+    }
+}
+
 // One grid, one step: block b runs tile xcd_logical(b).  Five waves per
 // SIMD fit in 96 VGPRs without spilling (the persistent batch kernel would
 // spill, so it keeps four).
@@ -501,6 +840,20 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(5))) voi
 // loads and the grid's pointers stay in SGPRs.
 template <typename T> using Const = const __attribute__((address_space(4))) T;
 template <typename T> __device__ __forceinline__ Const<T> *as_const(const T *p) { return (Const<T> *)p; }
+
+// One grid, one step, 16-bit storage with two cells per lane.
+template <int TY, typename T>
+__global__ __launch_bounds__(TX *TY) void fused_step_pair(const StepArgs A)
+{
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int bx = b % A.tiles_x;
+    const int by = (b / A.tiles_x) % A.tiles_y;
+    const int bz = b / (A.tiles_x * A.tiles_y);
+    stage_lut<false>(A.lut, A.nmat, tid, TX * TY);
+    fused_tile_pair<TY, T>(A, bx, by, bz);
+}
 
 // Step arguments of batch instance d at its step n: the ring slot and H
 // buffer roles follow n exactly as a single grid's do after n steps.
@@ -538,8 +891,11 @@ __device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, Step
 // block b runs work items b, b + gridDim.x, ... of the class's tile list
 // (instances in list order, each in z-chunk-major tile order).  Tiles of
 // instances outside their step window are skipped.
-template <int TY, typename T, bool EXACT>
-__global__ __launch_bounds__(TX *TY) void fused_batch(const BatchLaunch L)
+// The paired 16-bit form fits four waves per SIMD in 128 VGPRs without
+// spilling (left alone it takes 142 and three).
+template <int TY, typename T, bool EXACT, bool PAIR>
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4 : 1))) void fused_batch(
+    const BatchLaunch L)
 {
     const int tid = threadIdx.y * TX + threadIdx.x;
     Const<BatchClass> *C = as_const(L.cls);
@@ -564,7 +920,8 @@ __global__ __launch_bounds__(TX *TY) void fused_batch(const BatchLaunch L)
             stage_lut<EXACT>(A.lut, A.nmat, tid, TX * TY);
             staged = id;
         }
-        fused_tile<TY, T, EXACT>(A, bx, by, bz);
+        if constexpr (PAIR) fused_tile_pair<TY, T>(A, bx, by, bz);
+        else fused_tile<TY, T, EXACT>(A, bx, by, bz);
     }
 }
 
@@ -851,6 +1208,17 @@ hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s)
 {
     const unsigned nblk = (unsigned)a.tiles_x * a.tiles_y * a.tiles_z;
     const size_t lds = (size_t)a.nmat * (3 * sizeof(double) + 3 * sizeof(float));
+    if constexpr (sizeof(T) == 2) {
+        if (a.cpl == 2) {
+            switch (ty) {
+            case 2: fused_step_pair<2, T><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
+            case 4: fused_step_pair<4, T><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
+            case 8: fused_step_pair<8, T><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
+            default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
+    }
     switch (ty) {
     case 2: fused_step<2, T, EXACT><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
     case 4: fused_step<4, T, EXACT><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
@@ -875,41 +1243,41 @@ hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t
 
 namespace {
 
-template <int TY, typename T, bool EXACT>
+template <int TY, typename T, bool EXACT, bool PAIR>
 hipError_t launch_fused_batch_t(const BatchLaunch &l, int grid, hipStream_t s)
 {
-    fused_batch<TY, T, EXACT><<<grid, dim3(TX, TY), batch_fused_lds(), s>>>(l);
+    fused_batch<TY, T, EXACT, PAIR><<<grid, dim3(TX, TY), batch_fused_lds(), s>>>(l);
     return hipGetLastError();
 }
 
-template <typename T, bool EXACT>
+template <typename T, bool EXACT, bool PAIR>
 hipError_t launch_fused_batch_s(const BatchLaunch &l, int ty, int grid, hipStream_t s)
 {
     switch (ty) {
-    case 2: return launch_fused_batch_t<2, T, EXACT>(l, grid, s);
-    case 4: return launch_fused_batch_t<4, T, EXACT>(l, grid, s);
-    case 8: return launch_fused_batch_t<8, T, EXACT>(l, grid, s);
+    case 2: return launch_fused_batch_t<2, T, EXACT, PAIR>(l, grid, s);
+    case 4: return launch_fused_batch_t<4, T, EXACT, PAIR>(l, grid, s);
+    case 8: return launch_fused_batch_t<8, T, EXACT, PAIR>(l, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int TY, typename T, bool EXACT>
+template <int TY, typename T, bool EXACT, bool PAIR>
 int occupancy_t()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fused_batch<TY, T, EXACT>, TX * TY, batch_fused_lds()) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fused_batch<TY, T, EXACT, PAIR>, TX * TY,
+                                                     batch_fused_lds()) != hipSuccess)
         return 0;
     return nb;
 }
 
-template <typename T, bool EXACT>
+template <typename T, bool EXACT, bool PAIR>
 int occupancy_s(int ty)
 {
     switch (ty) {
-    case 2: return occupancy_t<2, T, EXACT>();
-    case 4: return occupancy_t<4, T, EXACT>();
-    case 8: return occupancy_t<8, T, EXACT>();
+    case 2: return occupancy_t<2, T, EXACT, PAIR>();
+    case 4: return occupancy_t<4, T, EXACT, PAIR>();
+    case 8: return occupancy_t<8, T, EXACT, PAIR>();
     default: return 0;
     }
 }
@@ -918,24 +1286,28 @@ int occupancy_s(int ty)
 
 size_t batch_fused_lds() { return (size_t)kMaxMaterials * (3 * sizeof(double) + 3 * sizeof(float)); }
 
-hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int storage, int grid, hipStream_t s)
+hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int cls, int grid, hipStream_t s)
 {
-    switch (storage) {
-    case kStoreF32: return launch_fused_batch_s<float, true>(l, ty, grid, s);
-    case kStoreF32Fast: return launch_fused_batch_s<float, false>(l, ty, grid, s);
-    case kStoreF16: return launch_fused_batch_s<__half, false>(l, ty, grid, s);
-    case kStoreBF16: return launch_fused_batch_s<__hip_bfloat16, false>(l, ty, grid, s);
+    switch (cls) {
+    case 0: return launch_fused_batch_s<float, true, false>(l, ty, grid, s);
+    case 1: return launch_fused_batch_s<__half, false, false>(l, ty, grid, s);
+    case 2: return launch_fused_batch_s<__hip_bfloat16, false, false>(l, ty, grid, s);
+    case 3: return launch_fused_batch_s<float, false, false>(l, ty, grid, s);
+    case 4: return launch_fused_batch_s<__half, false, true>(l, ty, grid, s);
+    case 5: return launch_fused_batch_s<__hip_bfloat16, false, true>(l, ty, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-int batch_fused_occupancy(int ty, int storage)
+int batch_fused_occupancy(int ty, int cls)
 {
-    switch (storage) {
-    case kStoreF32: return occupancy_s<float, true>(ty);
-    case kStoreF32Fast: return occupancy_s<float, false>(ty);
-    case kStoreF16: return occupancy_s<__half, false>(ty);
-    case kStoreBF16: return occupancy_s<__hip_bfloat16, false>(ty);
+    switch (cls) {
+    case 0: return occupancy_s<float, true, false>(ty);
+    case 1: return occupancy_s<__half, false, false>(ty);
+    case 2: return occupancy_s<__hip_bfloat16, false, false>(ty);
+    case 3: return occupancy_s<float, false, false>(ty);
+    case 4: return occupancy_s<__half, false, true>(ty);
+    case 5: return occupancy_s<__hip_bfloat16, false, true>(ty);
     default: return 0;
     }
 }

@@ -64,6 +64,8 @@ typedef struct
     float scale_e;      /* 16-bit storage: E stored as E / scale_e (0 = auto) */
     float scale_d;      /* 16-bit storage: D stored as D / (eps0 scale_d) (0 = auto) */
     float scale_h;      /* 16-bit storage: H stored as eta0 H / scale_h (0 = auto) */
+    int lane_cells;     /* 16-bit storage: cells per lane of the fused kernel (1 or 2; 0 = auto: 2 unless
+                           128-cell tiles leave clearly more lanes idle than 64-cell ones) */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;
@@ -212,7 +214,7 @@ typedef struct
     size_t largest_free;        /* largest region an instance can still get */
     int tile_y, kchunk;         /* fused launch shape */
     int graph_steps;            /* steps per captured graph */
-    int grid[4];                /* persistent blocks per fused launch, per storage form */
+    int grid[4];                /* persistent blocks per fused launch, per storage form (largest) */
     int tiles[4];               /* fused tiles per step of the running grids, per storage form */
     long long cells[4];         /* cells of the running grids, per storage form */
 } FdtdBatchInfo;

@@ -62,6 +62,7 @@ def parse():
                     help="field storage (f32 reproduces the reference's arithmetic bit for bit)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
@@ -137,6 +138,10 @@ def main():
     ctx.set_probes(w.probes, total)
     if w.slice_every:
         ctx.set_slices(w.slice_k, w.slice_every, total // w.slice_every + 1)
+    tuned = None
+    if not a.no_autotune:
+        # untimed: the autotuner runs each launch shape a few steps, then resets the grid to step 0
+        tuned = ctx.autotune(trial_steps=4, stream=stream.handle)
     ctx.run(a.warmup, stream.handle)
     stream.sync()
     ctx.set_timing(a.steps)
@@ -176,7 +181,8 @@ def main():
                  else "synthetic (seeded phantom)"),
         "config": {"workload": w.name, "grid": [nx, ny, nz], "cells": cells, "materials": int(w.table.shape[0]),
                    "sources": int(w.sources.shape[0]), "probes": int(w.probes.shape[0]),
-                   "slice_every": w.slice_every, "launch": shape, "parallelism": "single GPU"},
+                   "slice_every": w.slice_every, "launch": shape, "autotuned": tuned is not None,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fused_step", "kernel_avg_ms": round(kms / launches, 5),

@@ -48,6 +48,7 @@ EXPORTS = {
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
+                   "fdtd_ctx_autotune",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
@@ -168,6 +169,7 @@ def lib():
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
         L.fdtd_ctx_set_energy.argtypes = [vp, i, i]
         L.fdtd_ctx_save.argtypes = [vp, C.c_char_p]
+        L.fdtd_ctx_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_ctx_load.argtypes = [vp, C.c_char_p]
         L.fdtd_energy_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
@@ -293,6 +295,12 @@ class Context:
 
     def set_slices(self, k, interval, capacity):
         check(lib().fdtd_ctx_set_slices(self._h, k, interval, capacity), "set_slices")
+
+    def autotune(self, trial_steps=4, stream=None):
+        """(tile_y, kchunk, lane_cells) picked by timing each shape; the context returns to step 0."""
+        v = [C.c_int() for _ in range(3)]
+        check(lib().fdtd_ctx_autotune(self._h, trial_steps, stream, *[C.byref(x) for x in v]), "autotune")
+        return tuple(x.value for x in v)
 
     def save(self, path):
         check(lib().fdtd_ctx_save(self._h, str(path).encode()), "save")

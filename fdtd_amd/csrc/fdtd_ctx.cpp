@@ -39,6 +39,7 @@ struct fdtd_ctx {
     size_t esz;                          // bytes per stored element
     int ty, kchunk, tiles_x, tiles_y, tiles_z;
     int cpl = 1;                         // cells per lane of the fused kernel
+    bool fields_set = false;             // set_field or load gave non-zero state
 
     long long guard = 0;                 // elements before/after the field arrays and the ids
     char *blob = nullptr;                // all field arrays
@@ -132,6 +133,16 @@ int upload_lut(fdtd_ctx *c)
     return 0;
 }
 
+void set_shape(fdtd_ctx *c, int ty, int kchunk, int cpl)
+{
+    c->ty = ty;
+    c->kchunk = std::min(kchunk, c->nz);
+    c->cpl = cpl;
+    c->tiles_x = (c->nx + 64 * cpl - 1) / (64 * cpl);
+    c->tiles_y = (c->ny + ty - 1) / ty;
+    c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
+}
+
 hipError_t upload_murx(fdtd_ctx *c, const float *rpx0, const float *rpxEnd)
 {
     const size_t n = (size_t)c->ny * c->nz;
@@ -189,12 +200,10 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     // orders): 64x4 tiles marching 4 planes, identity order, within 1-3% of
     // every other shape at both sizes.
     c->ty = e.tile_y ? e.tile_y : 4;
-    // 16-bit fields: two cells per lane (one 32-bit access per stream) where the width suits 128-cell tiles
-    c->cpl = auto_lane_cells(c->nx, c->esz, e.lane_cells);
-    c->tiles_x = (c->nx + 64 * c->cpl - 1) / (64 * c->cpl);
-    c->tiles_y = (c->ny + c->ty - 1) / c->ty;
     c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : std::min(4, c->nz);
-    c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
+    // 16-bit fields: two cells per lane (one 32-bit access per stream) where that pays
+    c->cpl = auto_lane_cells_alone(c->nx, c->ny, c->nz, c->ty, c->kchunk, c->esz, e.lane_cells);
+    set_shape(c, c->ty, c->kchunk, c->cpl);
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H
     c->guard = fused_guard(c->pitch, c->plane);
@@ -377,6 +386,7 @@ extern "C" int fdtd_ctx_set_field(fdtd_ctx *c, int field, int level, const float
     fdtd_to_storage_pitched(c->storage, host, c->nx, (size_t)c->ny * c->nz, c->pitch, field_factor(c, field), buf.data());
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(field_dev(c, field, level), buf.data(), buf.size(), hipMemcpyHostToDevice));
+    c->fields_set = true;
     return 0;
 }
 
@@ -712,5 +722,68 @@ extern "C" int fdtd_ctx_load(fdtd_ctx *c, const char *path)
     c->sE = h.sE; c->sD = h.sD; c->sH = h.sH;
     c->scales_set = h.scales_set != 0;
     c->lut_dirty = true;
+    c->fields_set = true;
+    return 0;
+}
+
+// Times each launch shape on the context's own grid for trial_steps steps
+// (after two untimed ones), keeps the fastest and returns the context to
+// its zero state at step 0.
+extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t stream, int *tile_y, int *kchunk,
+                                 int *lane_cells)
+{
+    if (!c || trial_steps < 1) return fdtd_fail(FDTD_EINVAL, "bad autotune");
+    if (c->n != 0 || c->fields_set) return fdtd_fail(FDTD_ESTATE, "autotune needs an unstepped context with zero fields");
+    HIP_TRY(hipDeviceSynchronize());
+    const int slice_interval = c->slice_interval, energy_interval = c->energy_interval, tcap = c->tcap;
+    c->slice_interval = 0;
+    c->energy_interval = 0;
+    c->tcap = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipError_t e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    int rc = e == hipSuccess ? 0 : fdtd_hip_fail(e, "hipEventCreate");
+    const int tys[3] = {2, 4, 8}, kcs[3] = {4, 8, 16};
+    const int ncpl = c->esz == 2 ? 2 : 1;
+    float best = 0.0f;
+    int bty = c->ty, bkc = c->kchunk, bcpl = c->cpl;
+    for (int a = 0; a < 3 && !rc; a++)
+        for (int b = 0; b < 3 && !rc; b++)
+            for (int q = 1; q <= ncpl && !rc; q++) {
+                set_shape(c, tys[a], kcs[b], q);
+                rc = fdtd_run(c, 2, stream);
+                if (rc) break;
+                e = hipEventRecord(t0, stream);
+                if (e == hipSuccess) rc = fdtd_run(c, trial_steps, stream);
+                if (!rc && e == hipSuccess) e = hipEventRecord(t1, stream);
+                if (!rc && e == hipSuccess) e = hipEventSynchronize(t1);
+                float ms = 0.0f;
+                if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+                if (!rc && e != hipSuccess) rc = fdtd_hip_fail(e, "autotune timing");
+                if (!rc && (best == 0.0f || ms < best)) {
+                    best = ms;
+                    bty = tys[a];
+                    bkc = kcs[b];
+                    bcpl = q;
+                }
+            }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    // back to the zero state at step 0
+    e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemset(c->blob, 0, (24 * (c->Np + kArraySkew) + 2 * (size_t)c->guard) * c->esz);
+    if (!rc && e != hipSuccess) rc = fdtd_hip_fail(e, "autotune reset");
+    c->n = 0;
+    c->hcur = 0;
+    c->probe_rows = 0;
+    c->slice_interval = slice_interval;
+    c->energy_interval = energy_interval;
+    c->tcap = tcap;
+    c->tused = 0;
+    set_shape(c, bty, bkc, bcpl);
+    if (rc) return rc;
+    if (tile_y) *tile_y = bty;
+    if (kchunk) *kchunk = bkc;
+    if (lane_cells) *lane_cells = bcpl;
     return 0;
 }

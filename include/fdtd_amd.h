@@ -150,6 +150,13 @@ int fdtd_probe_read(fdtd_ctx *ctx, double *host, int max_rows);
  * (slices x 9 fields x nx*ny floats) up to max_slices. */
 int fdtd_slices_read(fdtd_ctx *ctx, int *steps, float *data, int max_slices);
 
+/* Autotuner: times every launch shape (tile rows {2,4,8} x z planes per
+ * block {4,8,16}, x cells per lane {1,2} under 16-bit storage) on this
+ * grid for trial_steps steps each, keeps the fastest and returns the
+ * context to its zero state at step 0 (probe rows and captures restart).
+ * Needs a context that has not stepped and whose fields were not set. */
+int fdtd_ctx_autotune(fdtd_ctx *ctx, int trial_steps, hipStream_t stream, int *tile_y, int *kchunk, int *lane_cells);
+
 /* The launch shape chosen for the context's fused kernel. */
 int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, int *blocks);
 

@@ -50,3 +50,31 @@ def test_pairs_equal_single_cells(storage, shape, kchunk):
         assert np.array_equal(a[key].view(np.uint32), b[key].view(np.uint32)), f"{F.FIELDS[key[0]]} level {key[1]}: {first_diff(a[key], b[key])}"
     assert np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
     assert np.isfinite(pa).all()
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F32, F.STORE_F16])
+def test_ctx_autotune_restores_state(storage):
+    shape = (70, 40, 33)
+    rng = np.random.default_rng(9)
+    ids, table = random_materials(rng, shape, 5)
+    src = random_sources(rng, shape, 2)
+
+    def make():
+        ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
+        ctx.set_materials(ids, table)
+        ctx.set_sources(src, _jz())
+        ctx.set_probes(np.asarray([[8, 35, 20, 16]], np.int32), 30)
+        return ctx
+
+    a = make()
+    ty, kc, lc = a.autotune(trial_steps=2)
+    assert ty in (2, 4, 8) and kc in (4, 8, 16) and lc in ((1, 2) if storage == F.STORE_F16 else (1,))
+    assert a.n == 0
+    a.run(30)
+    b = make()
+    b.run(30)
+    for f in range(9):
+        assert np.array_equal(a.get(f).view(np.uint32), b.get(f).view(np.uint32)), F.FIELDS[f]
+    assert np.array_equal(a.probes().view(np.uint64), b.probes().view(np.uint64))
+    with pytest.raises(F.FdtdError, match="-3"):
+        a.autotune()

@@ -93,7 +93,8 @@ class FdtdField(C.Structure):
 class FdtdExt(C.Structure):
     _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
                 ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int), ("storage", C.c_int),
-                ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float), ("lane_cells", C.c_int)]
+                ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float), ("lane_cells", C.c_int),
+                ("tile_x", C.c_int)]
 
 
 class FdtdInstance(C.Structure):
@@ -247,7 +248,7 @@ class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
     def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0,
-                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0):
+                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))
@@ -256,6 +257,7 @@ class Context:
         ext.storage = storage
         ext.scale_e, ext.scale_d, ext.scale_h = scales
         ext.lane_cells = lane_cells
+        ext.tile_x = tile_x
         h = C.c_void_p()
         self._params = params
         check(L.fdtd_ctx_create(C.byref(params), C.byref(ext), C.byref(h)), "fdtd_ctx_create")

@@ -242,6 +242,7 @@ int fdtd_check_ext(const FdtdExt &e)
         return fdtd_fail(FDTD_EINVAL, "storage %d", e.storage);
     if (!(e.scale_e >= 0.0f && e.scale_d >= 0.0f && e.scale_h >= 0.0f)) return fdtd_fail(FDTD_EINVAL, "negative scale");
     if (e.lane_cells < 0 || e.lane_cells > 2) return fdtd_fail(FDTD_EINVAL, "lane_cells %d", e.lane_cells);
+    if (e.tile_x != 0 && e.tile_x != 64 && e.tile_x != 128) return fdtd_fail(FDTD_EINVAL, "tile_x %d", e.tile_x);
     return 0;
 }
 

@@ -39,6 +39,7 @@ struct fdtd_ctx {
     size_t esz;                          // bytes per stored element
     int ty, kchunk, tiles_x, tiles_y, tiles_z;
     int cpl = 1;                         // cells per lane of the fused kernel
+    int txw = 64;                        // threads per tile row (one cell per lane)
     bool fields_set = false;             // set_field or load gave non-zero state
 
     long long guard = 0;                 // elements before/after the field arrays and the ids
@@ -133,12 +134,14 @@ int upload_lut(fdtd_ctx *c)
     return 0;
 }
 
-void set_shape(fdtd_ctx *c, int ty, int kchunk, int cpl)
+void set_shape(fdtd_ctx *c, int ty, int kchunk, int cpl, int txw = 64)
 {
     c->ty = ty;
     c->kchunk = std::min(kchunk, c->nz);
     c->cpl = cpl;
-    c->tiles_x = (c->nx + 64 * cpl - 1) / (64 * cpl);
+    c->txw = cpl == 2 ? 64 : txw;
+    const int width = c->cpl == 2 ? 128 : c->txw;
+    c->tiles_x = (c->nx + width - 1) / width;
     c->tiles_y = (c->ny + ty - 1) / ty;
     c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
 }
@@ -165,6 +168,7 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->storage = FDTD_STORE_F32;
     ext->scale_e = ext->scale_d = ext->scale_h = 0.0f;
     ext->lane_cells = 0;
+    ext->tile_x = 0;
     return 0;
 }
 
@@ -203,7 +207,7 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : std::min(4, c->nz);
     // 16-bit fields: two cells per lane (one 32-bit access per stream) where that pays
     c->cpl = auto_lane_cells_alone(c->nx, c->ny, c->nz, c->ty, c->kchunk, c->esz, e.lane_cells);
-    set_shape(c, c->ty, c->kchunk, c->cpl);
+    set_shape(c, c->ty, c->kchunk, c->cpl, e.tile_x ? e.tile_x : 64);
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H
     c->guard = fused_guard(c->pitch, c->plane);
@@ -448,6 +452,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.guard = (unsigned)c->guard;
     a.mur_x = fused_mur_x(c->nx, c->kchunk, 64 * c->cpl);
     a.cpl = c->cpl;
+    a.txw = c->txw;
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
     const bool timed = c->tused < c->tcap;
@@ -563,7 +568,7 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
 extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int *kchunk, int *blocks)
 {
     if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
-    if (tile_x) *tile_x = 64 * c->cpl;
+    if (tile_x) *tile_x = c->cpl == 2 ? 128 : c->txw;
     if (tile_y) *tile_y = c->ty;
     if (kchunk) *kchunk = c->kchunk;
     if (blocks) *blocks = c->tiles_x * c->tiles_y * c->tiles_z;
@@ -744,13 +749,16 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? 0 : fdtd_hip_fail(e, "hipEventCreate");
     const int tys[3] = {2, 4, 8}, kcs[3] = {4, 8, 16};
-    const int ncpl = c->esz == 2 ? 2 : 1;
+    // (cells per lane, threads per row): single cells 64 or 128 wide, 16-bit pairs 128 cells wide
+    const int forms[3][2] = {{1, 64}, {1, 128}, {2, 64}};
+    const int nform = c->esz == 2 ? 3 : 2;
     float best = 0.0f;
-    int bty = c->ty, bkc = c->kchunk, bcpl = c->cpl;
+    int bty = c->ty, bkc = c->kchunk, bcpl = c->cpl, btxw = c->txw;
     for (int a = 0; a < 3 && !rc; a++)
         for (int b = 0; b < 3 && !rc; b++)
-            for (int q = 1; q <= ncpl && !rc; q++) {
-                set_shape(c, tys[a], kcs[b], q);
+            for (int f = 0; f < nform && !rc; f++) {
+                const int q = forms[f][0], w = forms[f][1];
+                set_shape(c, tys[a], kcs[b], q, w);
                 rc = fdtd_run(c, 2, stream);
                 if (rc) break;
                 e = hipEventRecord(t0, stream);
@@ -765,6 +773,7 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
                     bty = tys[a];
                     bkc = kcs[b];
                     bcpl = q;
+                    btxw = w;
                 }
             }
     if (t0) (void)hipEventDestroy(t0);
@@ -780,7 +789,7 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     c->energy_interval = energy_interval;
     c->tcap = tcap;
     c->tused = 0;
-    set_shape(c, bty, bkc, bcpl);
+    set_shape(c, bty, bkc, bcpl, btxw);
     if (rc) return rc;
     if (tile_y) *tile_y = bty;
     if (kchunk) *kchunk = bkc;

@@ -59,10 +59,11 @@ struct StepArgs {
     int xcd_remap;                  // 0: identity, 1: contiguous runs per XCD, 2: layer-interleaved
     unsigned guard;                 // addressable elements before and after every field array (and ids)
     // Mur x faces updated inside the fused kernel (fused_mur_x): 0 none, 1
-    // all four (Ey, Ez at i = 0 and nx-1), 2 all but Ez at i = nx-1.
+    // all four (Ey, Ez at i = 0 and nx-1), 2 the two at i = 0 only.
     int mur_x;
     const float4 *murx0, *murx1;    // per (k, j) coefficients of the x-low / x-high faces (mur_x_coefs)
     int cpl;                        // cells per lane: 1, or 2 for 16-bit storage (128-cell tiles)
+    int txw;                        // threads per tile row with one cell per lane: 64 or 128
 };
 
 // Coefficients {q, A, B, C} of the first-order Mur update of one x-face
@@ -75,10 +76,15 @@ struct StepArgs {
 void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, const float *rpx0, const float *rpxEnd,
                  float4 *out0, float4 *out1);
 
-// Which Mur x faces the fused kernel takes over: the x-high Ez face needs
-// E^{n+1}_z at i = nx-2, which lives in the previous tile when nx-1 is a
-// multiple of the tile width (64 cells, or 128 with two cells per lane);
-// that face then stays in the Mur kernel.
+// Which Mur x faces the fused kernel takes over.  The x-high faces need the
+// cell at i = nx-2 from the face's wave: E^{n+1}_z from the previous lane
+// through a wave shuffle, and E^n_y, E^n_z from the LDS column that lane
+// staged, read after the plane's last barrier (only the staging wave's own
+// program order keeps the next plane from overwriting it).  When nx-1 is a
+// multiple of the cells one wave covers (tile_cells: 64, or 128 with two
+// cells per lane; a 128-wide tile of one cell per lane is two waves of 64)
+// the face is the first cell of a wave, that neighbour belongs to another
+// wave or tile, and both x-high faces stay in the Mur kernel.
 // The fused kernel stages the coefficients of its z-chunk in LDS, so chunks
 // longer than kFusedMurMaxChunk planes leave the x faces to the Mur kernel.
 constexpr int kFusedMurMaxChunk = 16;
@@ -91,7 +97,7 @@ inline int fused_mur_x(int nx, int kchunk, int tile_cells)
 inline unsigned mur_skip_mask(int mur_x)
 {
     // segments 4, 5: Ey at i = 0, nx-1; 8, 9: Ez at i = 0, nx-1
-    return mur_x == 1 ? (1u << 4 | 1u << 5 | 1u << 8 | 1u << 9) : mur_x == 2 ? (1u << 4 | 1u << 5 | 1u << 8) : 0u;
+    return mur_x == 1 ? (1u << 4 | 1u << 5 | 1u << 8 | 1u << 9) : mur_x == 2 ? (1u << 4 | 1u << 8) : 0u;
 }
 
 // Row pitch (elements) of the fused layout: rows start on 128-byte lines, so
@@ -107,7 +113,7 @@ inline int fused_pitch(int nx, size_t esz)
 
 // Guard (in elements) the fused kernel needs around every field array and
 // the id array so that its halo loads need no bounds checks: one plane, ten
-// rows and 256 elements (tiles are at most 8 rows high and 64 wide).
+// rows and 256 elements (tiles are at most 8 rows high and 128 cells wide).
 inline long long fused_guard(int pitch, long long plane) { return plane + 10LL * pitch + 256; }
 
 // Arguments of the 6-face first-order Mur kernel (fdtd_calculations.cu:216-428).

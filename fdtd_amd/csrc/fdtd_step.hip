@@ -190,13 +190,16 @@ __device__ __forceinline__ void stage_lut(const MatCoef *lut, int nmat, int tid,
     }
 }
 
-// One 64 x TY x kchunk tile (column tile (bx, by), z-chunk bz) of one step.
-// The coefficient table must be staged; the first LDS read of it follows a
-// barrier inside.  The tile's own LDS writes start at once, so a caller that
-// runs several tiles per block puts a barrier between them.
-template <int TY, typename T, bool EXACT>
+// One TXW x TY x kchunk tile (column tile (bx, by), z-chunk bz) of one step,
+// TXW = 64 (one wave per row) or 128 (two waves per row, which halves the
+// x-halo column reads per cell).  The coefficient table must be staged; the
+// first LDS read of it follows a barrier inside.  The tile's own LDS writes
+// start at once, so a caller that runs several tiles per block puts a
+// barrier between them.
+template <int TY, typename T, bool EXACT, int TXW = 64>
 __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, int bz)
 {
+    static_assert(TXW == 64 || TXW == 128, "tile width");
     static_assert(TY >= 2 && TY <= 8, "tile rows");
     static_assert(!EXACT || std::is_same<T, float>::value, "the exact form stores fp32");
     // EXACT reproduces the reference's arithmetic (double Debye sums); the
@@ -204,13 +207,13 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     constexpr bool exact = EXACT;
     constexpr uint32_t ES = sizeof(T);
     using X = Elem<T>;
-    __shared__ float sE[2][3][TY + 2][TX + 2];
-    __shared__ float sH[3][TY + 1][TX + 1];
-    __shared__ float sJunk[2][TX];
+    __shared__ float sE[2][3][TY + 2][TXW + 2];
+    __shared__ float sH[3][TY + 1][TXW + 1];
+    __shared__ float sJunk[2][TXW];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int i0 = bx * TX, j0 = by * TY;
+    const int i0 = bx * TXW, j0 = by * TY;
     const int k0 = bz * A.kchunk;
     const int k1 = min(k0 + A.kchunk, nz);
     const int i = i0 + tx, j = j0 + ty;
@@ -248,10 +251,11 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     const uint32_t c_rhi = gb + (uint32_t)(j0 + TY) * syb + (uint32_t)il * ES;
     const int side = tx >> 5;
     const int cr = min(tx & 31, TY + 1);             // LDS row of the gathered column cell
-    const int cl = side ? TX + 1 : 0;                 // LDS column of the gathered cell
-    const int ci = side ? i0 + TX : i0 - 1;
+    const int cl = side ? TXW + 1 : 0;                // LDS column of the gathered cell
+    const int ci = side ? i0 + TXW : i0 - 1;
     const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * ES;
-    const bool w_rlo = (ty == 0), w_rhi = (ty == TY - 1), w_col = (ty == 1);
+    // the column gather is the first wave of row 1 (wave-uniform for both widths)
+    const bool w_rlo = (ty == 0), w_rhi = (ty == TY - 1), w_col = (ty == 1) && (tx < 64);
 
     const float chx = A.chx, chy = A.chy, chz = A.chz;
     const float cdx = A.cdx, cdy = A.cdy, cdz = A.cdz;
@@ -285,7 +289,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
         o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
         o.id = __builtin_nontemporal_load(ids + ob / ES);
-        // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TX, row cr): Hy, Hz
+        // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TXW, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;
         if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
         if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
@@ -298,10 +302,10 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     __shared__ float4 sMurC[2][kMurK][TY];
     __shared__ float sMurT[2][TY];
     const bool mur_lo = A.mur_x && i0 == 0;
-    const bool mur_hi = A.mur_x && i0 <= nx - 1 && nx - 1 < i0 + TX;
+    const bool mur_hi = A.mur_x == 1 && i0 <= nx - 1 && nx - 1 < i0 + TXW;
     if (mur_lo || mur_hi) {
-        const int tid = ty * TX + tx, nk = k1 - k0;
-        for (int q = tid; q < 2 * nk * TY; q += TX * TY) {
+        const int tid = ty * TXW + tx, nk = k1 - k0;
+        for (int q = tid; q < 2 * nk * TY; q += TXW * TY) {
             const int side = q / (nk * TY), r = q - side * nk * TY, kk = r / TY, row = r - kk * TY;
             const long long at = (long long)(k0 + kk) * ny + min(j0 + row, ny - 1);
             sMurC[side][kk][row] = side ? A.murx1[at] : A.murx0[at];
@@ -459,7 +463,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
                     eyn = c.x * ((c.y * sn + c.z * sn) - c.w * eyk);
                     sty = true;
                 }
-                if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
+                if (j >= 1 && j < ny - 1 && k < nz - 1) {
                     const float tn = k >= 1 ? ezm : sMurT[1][ty];
                     ezn = c.x * ((c.y * tn + c.z * sE[cur][2][ty + 1][tx]) - c.w * ezk);
                     stz = true;
@@ -626,7 +630,7 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
 
     // Mur x faces, staged as in fused_tile.
     const bool mur_lo = A.mur_x && i0 == 0;
-    const bool mur_hi = A.mur_x && i0 <= nx - 1 && nx - 1 < i0 + TC;
+    const bool mur_hi = A.mur_x == 1 && i0 <= nx - 1 && nx - 1 < i0 + TC;
     if (mur_lo || mur_hi) {
         const int tid = ty * TX + tx, nk = k1 - k0;
         for (int q = tid; q < 2 * nk * TY; q += TX * TY) {
@@ -799,7 +803,7 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
                     eyn[h] = cm.x * ((cm.y * sn + cm.z * sn) - cm.w * eyk[h]);
                     sty[h] = true;
                 }
-                if (A.mur_x == 1 && j >= 1 && j < ny - 1 && k < nz - 1) {
+                if (j >= 1 && j < ny - 1 && k < nz - 1) {
                     const float tn = k >= 1 ? (h ? X::rnd(ezn[0]) : ezm) : sMurT[1][ty];
                     ezn[h] = cm.x * ((cm.y * tn + cm.z * sE[cur][2][ty + 1][c - 1]) - cm.w * ezk[h]);
                     stz[h] = true;
@@ -822,17 +826,17 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
 // One grid, one step: block b runs tile xcd_logical(b).  Five waves per
 // SIMD fit in 96 VGPRs without spilling (the persistent batch kernel would
 // spill, so it keeps four).
-template <int TY, typename T, bool EXACT>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(5))) void fused_step(const StepArgs A)
+template <int TY, typename T, bool EXACT, int TXW>
+__global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) void fused_step(const StepArgs A)
 {
-    const int tid = threadIdx.y * TX + threadIdx.x;
+    const int tid = threadIdx.y * TXW + threadIdx.x;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
     const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
     const int bx = b % A.tiles_x;
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);
-    stage_lut<EXACT>(A.lut, A.nmat, tid, TX * TY);
-    fused_tile<TY, T, EXACT>(A, bx, by, bz);
+    stage_lut<EXACT>(A.lut, A.nmat, tid, TXW * TY);
+    fused_tile<TY, T, EXACT, TXW>(A, bx, by, bz);
 }
 
 // Batch tables are read through the constant address space: they do not
@@ -1219,10 +1223,19 @@ hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s)
             return hipGetLastError();
         }
     }
+    if (a.txw == 128) {
+        switch (ty) {
+        case 2: fused_step<2, T, EXACT, 128><<<nblk, dim3(128, 2), lds, s>>>(a); break;
+        case 4: fused_step<4, T, EXACT, 128><<<nblk, dim3(128, 4), lds, s>>>(a); break;
+        case 8: fused_step<8, T, EXACT, 128><<<nblk, dim3(128, 8), lds, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (ty) {
-    case 2: fused_step<2, T, EXACT><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
-    case 4: fused_step<4, T, EXACT><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
-    case 8: fused_step<8, T, EXACT><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
+    case 2: fused_step<2, T, EXACT, 64><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
+    case 4: fused_step<4, T, EXACT, 64><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
+    case 8: fused_step<8, T, EXACT, 64><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

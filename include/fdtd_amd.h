@@ -66,6 +66,7 @@ typedef struct
     float scale_h;      /* 16-bit storage: H stored as eta0 H / scale_h (0 = auto) */
     int lane_cells;     /* 16-bit storage: cells per lane of the fused kernel (1 or 2; 0 = auto: 2 unless
                            128-cell tiles leave clearly more lanes idle than 64-cell ones) */
+    int tile_x;         /* threads per tile row with one cell per lane: 0 = 64, or 64, 128 */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;
@@ -151,7 +152,8 @@ int fdtd_probe_read(fdtd_ctx *ctx, double *host, int max_rows);
 int fdtd_slices_read(fdtd_ctx *ctx, int *steps, float *data, int max_slices);
 
 /* Autotuner: times every launch shape (tile rows {2,4,8} x z planes per
- * block {4,8,16}, x cells per lane {1,2} under 16-bit storage) on this
+ * block {4,8,16} x tile rows of 64 or 128 threads, and two cells per lane
+ * under 16-bit storage) on this
  * grid for trial_steps steps each, keeps the fastest and returns the
  * context to its zero state at step 0 (probe rows and captures restart).
  * Needs a context that has not stepped and whose fields were not set. */

@@ -118,28 +118,42 @@ def test_legacy_entry_points_bit_exact(debye):
 
 
 FUSED_CASES = [
-    # shape, tile_y, kchunk, debye, jz mode
-    ((37, 29, 23), 4, 0, O.DEBYE_FIXED, F.JZ_STEP),
-    ((37, 29, 23), 2, 5, O.DEBYE_REF, F.JZ_HOLD3),
-    ((64, 64, 64), 8, 7, O.DEBYE_FIXED, F.JZ_STEP),
-    ((130, 17, 9), 2, 3, O.DEBYE_FIXED, F.JZ_HOLD3),
-    ((5, 6, 7), 4, 0, O.DEBYE_FIXED, F.JZ_STEP),
-    ((65, 5, 40), 4, 40, O.DEBYE_REF, F.JZ_STEP),
+    # shape, tile_y, kchunk, debye, jz mode, threads per tile row
+    ((37, 29, 23), 4, 0, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((37, 29, 23), 2, 5, O.DEBYE_REF, F.JZ_HOLD3, 64),
+    ((64, 64, 64), 8, 7, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((130, 17, 9), 2, 3, O.DEBYE_FIXED, F.JZ_HOLD3, 64),
+    ((5, 6, 7), 4, 0, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((65, 5, 40), 4, 40, O.DEBYE_REF, F.JZ_STEP, 64),
+    ((130, 17, 9), 4, 3, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    ((200, 11, 13), 8, 4, O.DEBYE_REF, F.JZ_HOLD3, 128),
+    ((65, 9, 12), 2, 0, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    ((129, 7, 10), 4, 16, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    ((37, 29, 23), 4, 0, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    # x-high face on the first cell of a wave (nx-1 a multiple of 64): the
+    # x-high Mur faces stay in the Mur kernel; tall tiles put most rows' face
+    # neighbour in another wave's staged column
+    ((65, 20, 30), 8, 0, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((129, 9, 12), 4, 5, O.DEBYE_REF, F.JZ_STEP, 64),
+    ((193, 9, 12), 8, 4, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    # face inside the second wave of a 128-wide tile (folded into the tile)
+    ((100, 13, 11), 8, 0, O.DEBYE_FIXED, F.JZ_HOLD3, 128),
 ]
 
 
-@pytest.mark.parametrize("shape,tile_y,kchunk,debye,jz_mode", FUSED_CASES)
-def test_fused_step_bit_exact(shape, tile_y, kchunk, debye, jz_mode):
+@pytest.mark.parametrize("shape,tile_y,kchunk,debye,jz_mode,tile_x", FUSED_CASES)
+def test_fused_step_bit_exact(shape, tile_y, kchunk, debye, jz_mode, tile_x):
     """Fused H/D/E kernel + Mur faces over random state, partial tiles,
     several launch shapes: every field of every slot bit-equal."""
-    print(f"fused {shape} ty={tile_y} kc={kchunk} debye={debye} jz={jz_mode}", flush=True)
+    print(f"fused {shape} ty={tile_y} kc={kchunk} debye={debye} jz={jz_mode} tx={tile_x}", flush=True)
     rng = np.random.default_rng(sum(shape) + tile_y)
     ids, table = random_materials(rng, shape)
     rps = random_rp(rng, shape)
     src = random_sources(rng, shape, 4)
     jz = rng.uniform(-2, 2, 20).astype(np.float32)
     st = random_state(rng, shape)
-    ctx = F.Context(_params(shape), debye=debye, jz_mode=jz_mode, tile_y=tile_y, kchunk=kchunk)
+    ctx = F.Context(_params(shape), debye=debye, jz_mode=jz_mode, tile_y=tile_y, kchunk=kchunk, tile_x=tile_x)
+    assert ctx.launch_shape()["tile_x"] == tile_x
     ctx.set_materials(ids, table)
     ctx.set_rp(rps)
     ctx.set_sources(src, jz)

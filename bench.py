@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--time-every", type=int, default=16,
+                    help="HIP events around every N-th fused launch of the timed steps (each pair costs the "
+                         "stream a few microseconds)")
     ap.add_argument("--workload", default="plain_256",
                     choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken"])
     ap.add_argument("--phantom", default=os.path.join(ROOT, "data", "riken_ids.npz"),
@@ -144,7 +147,7 @@ def main():
         tuned = ctx.autotune(trial_steps=4, stream=stream.handle)
     ctx.run(a.warmup, stream.handle)
     stream.sync()
-    ctx.set_timing(a.steps)
+    ctx.set_timing(a.steps, every=max(1, a.time_every))
 
     e0, e1 = hip.Event(), hip.Event()
     hip.sync()
@@ -186,6 +189,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fused_step", "kernel_avg_ms": round(kms / launches, 5),
+                     "launches_timed": launches,
                      "bytes_per_cell": bpc},
         "probes_finite_nonzero": ok,
     }

@@ -165,7 +165,7 @@ def lib():
         L.fdtd_probe_read.argtypes = [vp, vp, i]
         L.fdtd_slices_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
-        L.fdtd_ctx_set_timing.argtypes = [vp, i]
+        L.fdtd_ctx_set_timing.argtypes = [vp, i, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
         L.fdtd_ctx_set_energy.argtypes = [vp, i, i]
@@ -367,8 +367,10 @@ class Context:
             check(got, "slices_read")
         return steps[:got], data[:got]
 
-    def set_timing(self, capacity):
-        check(lib().fdtd_ctx_set_timing(self._h, capacity), "set_timing")
+    def set_timing(self, capacity, every=1):
+        """Time every `every`-th fused launch (the first included), at most
+        `capacity` of them."""
+        check(lib().fdtd_ctx_set_timing(self._h, capacity, every), "set_timing")
 
     def kernel_time(self):
         """(summed fused-kernel milliseconds, launches timed)."""

@@ -304,14 +304,13 @@ int graph_for(fdtd_batch *b, int mask, int g, hipGraphExec_t *out)
     }
     int grids[kBatchClasses];
     for (int c = 0; c < kBatchClasses; c++) grids[c] = (mask >> c & 1) ? fused_grid(b, c) : 0;
-    const int mur_grid = 4 * b->cus, probe_grid = 16;
+    const int mur_grid = 4 * b->cus;
     HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int s = 0; s < g && e == hipSuccess; s++) {
         for (int c = 0; c < kBatchClasses && e == hipSuccess; c++)
             if (mask >> c & 1) e = launch_fused_batch(launch_of(b, c, s), b->ty, c, grids[c], b->cap);
-        if (e == hipSuccess) e = launch_mur_batch(launch_of(b, 0, s), mur_grid, b->cap);
-        if (e == hipSuccess) e = launch_probe_batch(launch_of(b, 0, s), probe_grid, b->cap);
+        if (e == hipSuccess) e = launch_mur_batch(launch_of(b, 0, s), mur_grid, b->cap);  // and probe rows
     }
     if (e == hipSuccess) e = launch_batch_advance(&b->dev->sh, g, b->cap);
     hipGraph_t graph = nullptr;

@@ -77,6 +77,7 @@ struct fdtd_ctx {
 
     std::vector<hipEvent_t> tev;  // start/end pairs around fused launches
     int tcap = 0, tused = 0;
+    int tevery = 1, tseen = 0;    // time launches tseen % tevery == 0
 
     int n = 0;  // steps done
 };
@@ -455,7 +456,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.txw = c->txw;
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
-    const bool timed = c->tused < c->tcap;
+    const bool timed = c->tused < c->tcap && c->tseen++ % c->tevery == 0;
     if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
     hipError_t e = launch_fused_step(a, c->ty, c->storage, stream);
     if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step");
@@ -472,14 +473,14 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pitch = c->pitch; m.plane = c->plane;
     m.dt = c->dt; m.hx = c->dx; m.hy = c->dy; m.hz = c->dz; m.mu0 = c->mu0; m.eps0 = c->eps0;
     mur_segments(c->nx, c->ny, c->nz, m.seg_start, mur_skip_mask(a.mur_x));
-    e = launch_mur(m, c->storage, stream);
-    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces");
 
     c->n = n + 1;
     c->hcur = hn;
 
+    // The step's probe row is recorded by the Mur launch (newest levels).
+    ProbeArgs p;
+    p.nprobe = 0;
     if (c->nprobe > 0 && c->probe_rows < c->probe_cap) {
-        ProbeArgs p;
         for (int f = 0; f < 9; f++) {
             p.field[f] = field_dev(c, f, 0);
             p.factor[f] = field_factor(c, f);
@@ -490,9 +491,9 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
         p.plane = c->plane;
         p.out = c->probe_out;
         p.row = c->probe_rows++;
-        e = launch_probes(p, c->storage, stream);
-        if (e != hipSuccess) return fdtd_hip_fail(e, "probe_cells");
     }
+    e = launch_mur_probes(m, p, c->storage, stream);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces/probes");
     if (c->energy_interval > 0 && c->n % c->energy_interval == 0 && c->energy_rows < c->energy_cap) {
         EnergyArgs g;
         for (int q = 0; q < 3; q++) {
@@ -575,15 +576,17 @@ extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int 
     return 0;
 }
 
-extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity)
+extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity, int every)
 {
-    if (!c || capacity < 0) return fdtd_fail(FDTD_EINVAL, "bad timing capacity");
+    if (!c || capacity < 0 || every < 1) return fdtd_fail(FDTD_EINVAL, "bad timing capacity or stride");
     HIP_TRY(hipDeviceSynchronize());
     for (auto ev : c->tev) (void)hipEventDestroy(ev);
     c->tev.assign(2 * (size_t)capacity, nullptr);
     for (auto &ev : c->tev) HIP_TRY(hipEventCreate(&ev));
     c->tcap = capacity;
     c->tused = 0;
+    c->tevery = every;
+    c->tseen = 0;
     return 0;
 }
 
@@ -789,6 +792,7 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     c->energy_interval = energy_interval;
     c->tcap = tcap;
     c->tused = 0;
+    c->tseen = 0;
     set_shape(c, bty, bkc, bcpl, btxw);
     if (rc) return rc;
     if (tile_y) *tile_y = bty;

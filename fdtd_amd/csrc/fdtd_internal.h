@@ -155,6 +155,9 @@ hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
+// Mur faces and one probe row in one launch (p.nprobe = 0: faces only).  The
+// probe fields must be the newest levels, with E the Mur target a.exT..ezT.
+hipError_t launch_mur_probes(const MurArgs &a, const ProbeArgs &p, int storage, hipStream_t s);
 // Prefix sums of the 12 Mur face segments; segments whose bit is set in
 // `skip` get no cells.
 void mur_segments(int nx, int ny, int nz, int *seg_start, unsigned skip = 0);
@@ -262,7 +265,6 @@ struct BatchLaunch {
 size_t batch_fused_lds();
 hipError_t launch_fused_batch(const BatchLaunch &l, int ty, int cls, int grid, hipStream_t s);
 hipError_t launch_mur_batch(const BatchLaunch &l, int grid, hipStream_t s);
-hipError_t launch_probe_batch(const BatchLaunch &l, int grid, hipStream_t s);
 hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s);
 // Resident blocks of the fused batch kernel of a class per CU.
 int batch_fused_occupancy(int ty, int cls);

@@ -929,96 +929,111 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4
     }
 }
 
+// New value of face cell (i, j, k) of Mur segment s (fdtd_calculations.cu:
+// 237-427): E'_f = mur1(rp_face, rp_nb, E'_nb, E^n_nb, E^n_f) with the
+// neighbour one cell inward along the face normal.  Segments: 0, 1 Ex at
+// j = 0, ny-1; 2, 3 Ex at k = 0, nz-1; 4, 5 Ey at i = 0, nx-1; 6, 7 Ey at
+// k = 0, nz-1; 8, 9 Ez at i = 0, nx-1; 10, 11 Ez at j = 0, ny-1.  Every
+// E'_nb read lands on a cell no segment writes.
+template <typename T>
+__device__ __forceinline__ float mur_value(const MurArgs &M, int s, int i, int j, int k)
+{
+    using X = Elem<T>;
+    const T *exT = (const T *)M.exT, *eyT = (const T *)M.eyT, *ezT = (const T *)M.ezT;
+    const T *exS = (const T *)M.exS, *eyS = (const T *)M.eyS, *ezS = (const T *)M.ezS;
+    const int nx = M.nx, ny = M.ny;
+    const long long sy = M.pitch, sz = M.plane;
+    const float dt = M.dt, m0 = M.mu0, e0 = M.eps0;
+    const long long o = (long long)k * sz + (long long)j * sy + i;
+    auto G = [&](const T *p, long long q) { return X::ld(p, (uint32_t)(q * (long long)sizeof(T))); };
+    auto rpy = [&](const float *p, int jj) { return p[(long long)k * 2 * nx + (long long)jj * nx + i]; };
+    auto rpz = [&](const float *p, int kk) { return p[(long long)kk * ny * nx + (long long)j * nx + i]; };
+    auto rpx = [&](const float *p, int ii) { return p[(long long)k * ny * 2 + (long long)j * 2 + ii]; };
+    switch (s) {
+    case 0: return mur1(dt, M.hy, m0, e0, rpy(M.rpy0, 0), rpy(M.rpy0, 1), G(exT, o + sy), G(exS, o + sy), G(exS, o));
+    case 1: return mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, 1), rpy(M.rpyEnd, 0), G(exT, o - sy), G(exS, o - sy), G(exS, o));
+    case 2: return mur1(dt, M.hz, m0, e0, rpz(M.rpz0, 0), rpz(M.rpz0, 1), G(exT, o + sz), G(exS, o + sz), G(exS, o));
+    case 3: return mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, 1), rpz(M.rpzEnd, 0), G(exT, o - sz), G(exS, o - sz), G(exS, o));
+    case 4: return mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0), rpx(M.rpx0, 1), G(eyT, o + 1), G(eyS, o + 1), G(eyS, o));
+    // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
+    case 5: return mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1), rpx(M.rpxEnd, 0), G(eyS, o - 1), G(eyS, o - 1), G(eyS, o));
+    case 6: return mur1(dt, M.hz, m0, e0, rpz(M.rpz0, 0), rpz(M.rpz0, 1), G(eyT, o + sz), G(eyS, o + sz), G(eyS, o));
+    case 7: return mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, 1), rpz(M.rpzEnd, 0), G(eyT, o - sz), G(eyS, o - sz), G(eyS, o));
+    case 8: return mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0), rpx(M.rpx0, 1), G(ezT, o + 1), G(ezS, o + 1), G(ezS, o));
+    case 9: return mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1), rpx(M.rpxEnd, 0), G(ezT, o - 1), G(ezS, o - 1), G(ezS, o));
+    case 10: return mur1(dt, M.hy, m0, e0, rpy(M.rpy0, 0), rpy(M.rpy0, 1), G(ezT, o + sy), G(ezS, o + sy), G(ezS, o));
+    default: return mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, 1), rpy(M.rpyEnd, 0), G(ezT, o - sy), G(ezS, o - sy), G(ezS, o));
+    }
+}
+
+// The Mur segment that holds cell (i, j, k) of E component c (0 Ex, 1 Ey,
+// 2 Ez), or -1.  The ranges are the reference's (fdtd_calculations.cu:
+// 237-427); for each component they are disjoint.
+__device__ __forceinline__ int mur_segment_of(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (c == 0) {
+        if (i >= nx - 1) return -1;
+        if ((j == 0 || j == ny - 1) && k >= 1 && k < nz - 1) return j == 0 ? 0 : 1;
+        if ((k == 0 || k == nz - 1) && j >= 1 && j < ny - 1) return k == 0 ? 2 : 3;
+        return -1;
+    }
+    if (c == 1) {
+        if ((i == 0 || i == nx - 1) && j < ny - 1 && k >= 1 && k < nz - 1) return i == 0 ? 4 : 5;
+        if ((k == 0 || k == nz - 1) && i >= 1 && i < nx - 1 && j < ny - 1) return k == 0 ? 6 : 7;
+        return -1;
+    }
+    if ((i == 0 || i == nx - 1) && j >= 1 && j < ny - 1 && k < nz - 1) return i == 0 ? 8 : 9;
+    if ((j == 0 || j == ny - 1) && i >= 1 && i < nx - 1 && k < nz - 1) return j == 0 ? 10 : 11;
+    return -1;
+}
+
 // Face cell t (in segment order) of the 12 (component, face) Mur updates;
 // seg holds the 13 segment prefix sums (M.seg_start is not read).
 template <typename T, typename Seg>
 __device__ __forceinline__ void mur_cell(const MurArgs &M, Seg seg, int t)
 {
-    using X = Elem<T>;
-    T *exT = (T *)M.exT, *eyT = (T *)M.eyT, *ezT = (T *)M.ezT;
-    const T *exS = (const T *)M.exS, *eyS = (const T *)M.eyS, *ezS = (const T *)M.ezS;
     if (t >= seg[12]) return;
-    int s = 0;
+    // the last segment starting at or before t (prefix sums never decrease;
+    // no dynamic index into seg, which would copy the arguments to scratch)
+    int s = 0, base = 0;
 #pragma unroll
-    for (int q = 1; q < 12; q++) s += (t >= seg[q]) ? 1 : 0;
-    const int q = t - seg[s];
+    for (int q = 1; q < 12; q++)
+        if (t >= seg[q]) { s = q; base = seg[q]; }
+    const int q = t - base;
     const int nx = M.nx, ny = M.ny, nz = M.nz;
-    const long long sy = M.pitch, sz = M.plane;
-    const float dt = M.dt, m0 = M.mu0, e0 = M.eps0;
-    auto at = [&](int i, int j, int k) { return (long long)k * sz + (long long)j * sy + i; };
-    auto G = [&](const T *p, long long o) { return X::ld(p, (uint32_t)(o * (long long)sizeof(T))); };
-    auto P = [&](T *p, long long o, float v) { X::st(p, (uint32_t)(o * (long long)sizeof(T)), v); };
-    auto rpy = [&](const float *p, int i, int jj, int k) { return p[(long long)k * 2 * nx + (long long)jj * nx + i]; };
-    auto rpz = [&](const float *p, int i, int j, int kk) { return p[(long long)kk * ny * nx + (long long)j * nx + i]; };
-    auto rpx = [&](const float *p, int ii, int j, int k) { return p[(long long)k * ny * 2 + (long long)j * 2 + ii]; };
+    int i, j, k;
     switch (s) {
-    case 0: case 1: {  // Ex on y faces: i in [0,nx-1), k in [1,nz-1)
-        const int i = q % (nx - 1), k = 1 + q / (nx - 1);
-        if (s == 0) {
-            const long long o = at(i, 0, k);
-            P(exT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), G(exT, o + sy), G(exS, o + sy), G(exS, o)));
-        } else {
-            const long long o = at(i, ny - 1, k);
-            P(exT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), G(exT, o - sy), G(exS, o - sy), G(exS, o)));
-        }
-        break;
+    case 0: case 1: i = q % (nx - 1); k = 1 + q / (nx - 1); j = s == 0 ? 0 : ny - 1; break;   // i in [0,nx-1), k in [1,nz-1)
+    case 2: case 3: i = q % (nx - 1); j = 1 + q / (nx - 1); k = s == 2 ? 0 : nz - 1; break;   // i in [0,nx-1), j in [1,ny-1)
+    case 4: case 5: j = q % (ny - 1); k = 1 + q / (ny - 1); i = s == 4 ? 0 : nx - 1; break;   // j in [0,ny-1), k in [1,nz-1)
+    case 6: case 7: i = 1 + q % (nx - 2); j = q / (nx - 2); k = s == 6 ? 0 : nz - 1; break;   // i in [1,nx-1), j in [0,ny-1)
+    case 8: case 9: j = 1 + q % (ny - 2); k = q / (ny - 2); i = s == 8 ? 0 : nx - 1; break;   // j in [1,ny-1), k in [0,nz-1)
+    default: i = 1 + q % (nx - 2); k = q / (nx - 2); j = s == 10 ? 0 : ny - 1; break;         // i in [1,nx-1), k in [0,nz-1)
     }
-    case 2: case 3: {  // Ex on z faces: i in [0,nx-1), j in [1,ny-1)
-        const int i = q % (nx - 1), j = 1 + q / (nx - 1);
-        if (s == 2) {
-            const long long o = at(i, j, 0);
-            P(exT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), G(exT, o + sz), G(exS, o + sz), G(exS, o)));
-        } else {
-            const long long o = at(i, j, nz - 1);
-            P(exT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), G(exT, o - sz), G(exS, o - sz), G(exS, o)));
-        }
-        break;
+    const float v = mur_value<T>(M, s, i, j, k);
+    T *dst = (T *)(s < 4 ? M.exT : (s < 8 ? M.eyT : M.ezT));
+    const long long o = (long long)k * M.plane + (long long)j * M.pitch + i;
+    Elem<T>::st(dst, (uint32_t)(o * (long long)sizeof(T)), v);
+}
+
+// Probe value of field f (Dx Dy Dz Hx Hy Hz Ex Ey Ez) at (i, j, k) after this
+// step's Mur faces, read in the same launch as the faces are written: the
+// stored value, except for an E cell of a segment this launch updates, whose
+// new value is evaluated here from the same operands (so it equals, bit for
+// bit, what the face thread stores).
+template <typename T, typename Seg>
+__device__ __forceinline__ float probe_value(const MurArgs &M, Seg seg, const void *field, int f, int i, int j, int k)
+{
+    if (f >= 6) {
+        const int s = mur_segment_of(f - 6, i, j, k, M.nx, M.ny, M.nz);
+        bool active = false;
+#pragma unroll
+        for (int q = 0; q < 12; q++)
+            if (q == s) active = seg[q + 1] > seg[q];
+        if (active) return mur_value<T>(M, s, i, j, k);
     }
-    case 4: case 5: {  // Ey on x faces: j in [0,ny-1), k in [1,nz-1)
-        const int j = q % (ny - 1), k = 1 + q / (ny - 1);
-        if (s == 4) {
-            const long long o = at(0, j, k);
-            P(eyT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), G(eyT, o + 1), G(eyS, o + 1), G(eyS, o)));
-        } else {  // the x-high Ey face reads the source neighbour twice (fdtd_calculations.cu:323-325)
-            const long long o = at(nx - 1, j, k);
-            P(eyT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), G(eyS, o - 1), G(eyS, o - 1), G(eyS, o)));
-        }
-        break;
-    }
-    case 6: case 7: {  // Ey on z faces: i in [1,nx-1), j in [0,ny-1)
-        const int i = 1 + q % (nx - 2), j = q / (nx - 2);
-        if (s == 6) {
-            const long long o = at(i, j, 0);
-            P(eyT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpz0, i, j, 0), rpz(M.rpz0, i, j, 1), G(eyT, o + sz), G(eyS, o + sz), G(eyS, o)));
-        } else {
-            const long long o = at(i, j, nz - 1);
-            P(eyT, o, mur1(dt, M.hz, m0, e0, rpz(M.rpzEnd, i, j, 1), rpz(M.rpzEnd, i, j, 0), G(eyT, o - sz), G(eyS, o - sz), G(eyS, o)));
-        }
-        break;
-    }
-    case 8: case 9: {  // Ez on x faces: j in [1,ny-1), k in [0,nz-1)
-        const int j = 1 + q % (ny - 2), k = q / (ny - 2);
-        if (s == 8) {
-            const long long o = at(0, j, k);
-            P(ezT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpx0, 0, j, k), rpx(M.rpx0, 1, j, k), G(ezT, o + 1), G(ezS, o + 1), G(ezS, o)));
-        } else {
-            const long long o = at(nx - 1, j, k);
-            P(ezT, o, mur1(dt, M.hx, m0, e0, rpx(M.rpxEnd, 1, j, k), rpx(M.rpxEnd, 0, j, k), G(ezT, o - 1), G(ezS, o - 1), G(ezS, o)));
-        }
-        break;
-    }
-    default: {  // Ez on y faces: i in [1,nx-1), k in [0,nz-1)
-        const int i = 1 + q % (nx - 2), k = q / (nx - 2);
-        if (s == 10) {
-            const long long o = at(i, 0, k);
-            P(ezT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpy0, i, 0, k), rpy(M.rpy0, i, 1, k), G(ezT, o + sy), G(ezS, o + sy), G(ezS, o)));
-        } else {
-            const long long o = at(i, ny - 1, k);
-            P(ezT, o, mur1(dt, M.hy, m0, e0, rpy(M.rpyEnd, i, 1, k), rpy(M.rpyEnd, i, 0, k), G(ezT, o - sy), G(ezS, o - sy), G(ezS, o)));
-        }
-        break;
-    }
-    }
+    const long long o = (long long)k * M.plane + (long long)j * M.pitch + i;
+    return Elem<T>::ld((const T *)field, (uint32_t)(o * (long long)sizeof(T)));
 }
 
 template <typename T>
@@ -1027,8 +1042,44 @@ __global__ __launch_bounds__(kMurBlock) void mur_faces(const MurArgs M)
     mur_cell<T>(M, M.seg_start, blockIdx.x * kMurBlock + threadIdx.x);
 }
 
-// Mur faces of every live instance of a batch: block b runs work items b,
-// b + gridDim.x, ... of the shared Mur list (kMurBlock face cells each).
+// Mur faces, then (in the blocks after them) the step's probe row in the
+// same launch: the probe threads do not wait for the face threads, since
+// probe_value evaluates an updated face cell itself.
+template <typename T>
+__global__ __launch_bounds__(kMurBlock) void mur_probe(const MurArgs M, const ProbeArgs P)
+{
+    const int mb = (M.seg_start[12] + kMurBlock - 1) / kMurBlock;
+    if ((int)blockIdx.x < mb) {
+        mur_cell<T>(M, M.seg_start, blockIdx.x * kMurBlock + threadIdx.x);
+        return;
+    }
+    const int p = ((int)blockIdx.x - mb) * kMurBlock + (int)threadIdx.x;
+    if (p >= P.nprobe) return;
+    const int *q = P.probes + 4 * p;
+    const int f = q[0];
+    const float v = probe_value<T>(M, M.seg_start, P.field[f], f, q[1], q[2], q[3]);
+    P.out[(long long)P.row * P.nprobe + p] = (double)v * P.factor[f];
+}
+
+// Mur operands of batch instance d at its step n (ring slot n % 3 written,
+// (n - 1) % 3 the source).
+template <typename Inst>
+__device__ __forceinline__ void batch_mur_args(const Inst &d, int n, MurArgs &M)
+{
+    const int tt = n % 3, s1 = (n + 2) % 3;
+    M.exT = d.E[tt][0]; M.eyT = d.E[tt][1]; M.ezT = d.E[tt][2];
+    M.exS = d.E[s1][0]; M.eyS = d.E[s1][1]; M.ezS = d.E[s1][2];
+    M.rpx0 = d.rp[0]; M.rpy0 = d.rp[1]; M.rpz0 = d.rp[2];
+    M.rpxEnd = d.rp[3]; M.rpyEnd = d.rp[4]; M.rpzEnd = d.rp[5];
+    M.nx = d.nx; M.ny = d.ny; M.nz = d.nz; M.pitch = d.pitch; M.plane = d.plane;
+    M.dt = d.dt; M.hx = d.hx; M.hy = d.hy; M.hz = d.hz; M.mu0 = d.mu0; M.eps0 = d.eps0;
+}
+
+// Mur faces and probe rows of every live instance of a batch.  Block b runs
+// work items b, b + gridDim.x, ... of the shared Mur list (kMurBlock face
+// cells each), then probe records r = (instance, probe) in grid-stride
+// order; record r writes row n of the instance's series after its step n
+// (probe_value: no wait for the face threads).
 __global__ __launch_bounds__(kMurBlock) void mur_batch(const BatchLaunch L)
 {
     Const<BatchShared> *S = as_const(L.sh);
@@ -1040,14 +1091,8 @@ __global__ __launch_bounds__(kMurBlock) void mur_batch(const BatchLaunch L)
         Const<BatchInst> &d = as_const(L.inst)[S->mur_idx[q]];
         const int n = step - d.start;
         if (n < 0 || n >= d.nsteps) continue;
-        const int tt = n % 3, s1 = (n + 2) % 3;
         MurArgs M;
-        M.exT = d.E[tt][0]; M.eyT = d.E[tt][1]; M.ezT = d.E[tt][2];
-        M.exS = d.E[s1][0]; M.eyS = d.E[s1][1]; M.ezS = d.E[s1][2];
-        M.rpx0 = d.rp[0]; M.rpy0 = d.rp[1]; M.rpz0 = d.rp[2];
-        M.rpxEnd = d.rp[3]; M.rpyEnd = d.rp[4]; M.rpzEnd = d.rp[5];
-        M.nx = d.nx; M.ny = d.ny; M.nz = d.nz; M.pitch = d.pitch; M.plane = d.plane;
-        M.dt = d.dt; M.hx = d.hx; M.hy = d.hy; M.hz = d.hz; M.mu0 = d.mu0; M.eps0 = d.eps0;
+        batch_mur_args(d, n, M);
         Const<int> *seg = d.mur_seg;
         const int t = (blk - S->mur_start[q]) * kMurBlock + (int)threadIdx.x;
         switch (d.storage) {
@@ -1056,28 +1101,22 @@ __global__ __launch_bounds__(kMurBlock) void mur_batch(const BatchLaunch L)
         default: mur_cell<float>(M, seg, t); break;
         }
     }
-}
-
-// Probe records of a batch: record r = (instance, probe) writes row n of the
-// instance's series after its step n.
-__global__ __launch_bounds__(256) void probe_batch(const BatchLaunch L)
-{
-    const BatchShared *S = L.sh;
-    const int step = S->step + L.off;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < S->nrec; r += gridDim.x * blockDim.x) {
-        const int id = S->rec[r][0], p = S->rec[r][1];
+    const int nrec = S->nrec;
+    for (int r = blockIdx.x * kMurBlock + threadIdx.x; r < nrec; r += gridDim.x * kMurBlock) {
+        const int id = L.sh->rec[r][0], p = L.sh->rec[r][1];
         const BatchInst &d = L.inst[id];
         const int n = step - d.start;
         if (n < 0 || n >= d.nsteps || n >= d.probe_rows) continue;
-        const int *q = d.probes + 4 * p;
-        const int f = q[0];
-        const long long o = (long long)q[3] * d.plane + (long long)q[2] * d.pitch + q[1];
+        MurArgs M;
+        batch_mur_args(d, n, M);
+        const int *pq = d.probes + 4 * p;
+        const int f = pq[0];
         const void *base = f < 3 ? d.D[n % 3][f] : (f < 6 ? d.H[(n & 1) ^ 1][f - 3] : d.E[n % 3][f - 6]);
         float v;
         switch (d.storage) {
-        case kStoreF16: v = Elem<__half>::ld((const __half *)base, (uint32_t)(o * 2)); break;
-        case kStoreBF16: v = Elem<__hip_bfloat16>::ld((const __hip_bfloat16 *)base, (uint32_t)(o * 2)); break;
-        default: v = Elem<float>::ld((const float *)base, (uint32_t)(o * 4)); break;
+        case kStoreF16: v = probe_value<__half>(M, d.mur_seg, base, f, pq[1], pq[2], pq[3]); break;
+        case kStoreBF16: v = probe_value<__hip_bfloat16>(M, d.mur_seg, base, f, pq[1], pq[2], pq[3]); break;
+        default: v = probe_value<float>(M, d.mur_seg, base, f, pq[1], pq[2], pq[3]); break;
         }
         d.probe_out[(long long)n * d.nprobe + p] = (double)v * d.factor[f];
     }
@@ -1173,6 +1212,21 @@ hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s)
     case kStoreF32Fast: mur_faces<float><<<grid, kMurBlock, 0, s>>>(a); break;
     case kStoreF16: mur_faces<__half><<<grid, kMurBlock, 0, s>>>(a); break;
     case kStoreBF16: mur_faces<__hip_bfloat16><<<grid, kMurBlock, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mur_probes(const MurArgs &a, const ProbeArgs &p, int storage, hipStream_t s)
+{
+    const int mb = (a.seg_start[12] + kMurBlock - 1) / kMurBlock;
+    const int grid = mb + (p.nprobe + kMurBlock - 1) / kMurBlock;
+    if (grid <= 0) return hipSuccess;
+    switch (storage) {
+    case kStoreF32:
+    case kStoreF32Fast: mur_probe<float><<<grid, kMurBlock, 0, s>>>(a, p); break;
+    case kStoreF16: mur_probe<__half><<<grid, kMurBlock, 0, s>>>(a, p); break;
+    case kStoreBF16: mur_probe<__hip_bfloat16><<<grid, kMurBlock, 0, s>>>(a, p); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1331,11 +1385,6 @@ hipError_t launch_mur_batch(const BatchLaunch &l, int grid, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_probe_batch(const BatchLaunch &l, int grid, hipStream_t s)
-{
-    probe_batch<<<grid, 256, 0, s>>>(l);
-    return hipGetLastError();
-}
 
 hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s)
 {

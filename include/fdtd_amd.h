@@ -162,10 +162,13 @@ int fdtd_ctx_autotune(fdtd_ctx *ctx, int trial_steps, hipStream_t stream, int *t
 /* The launch shape chosen for the context's fused kernel. */
 int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, int *blocks);
 
-/* Records a HIP event pair around each fused-kernel launch of the next
- * `capacity` steps, on the stream the kernel runs on (0 disables);
- * fdtd_ctx_kernel_time waits for them and returns the summed duration. */
-int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity);
+/* Records a HIP event pair around every `every`-th fused-kernel launch
+ * (the first, then each `every` launches; every >= 1), up to `capacity`
+ * pairs, on the stream the kernel runs on (capacity 0 disables).  An event
+ * pair costs the stream a few microseconds, so sampling keeps the timed
+ * run's wall clock close to an untimed one.  fdtd_ctx_kernel_time waits
+ * for the recorded pairs and returns their summed duration. */
+int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity, int every);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
 
 /* ---- voxel phantoms (fdtd.cu:875-939) ------------------------------------ */

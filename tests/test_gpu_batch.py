@@ -25,12 +25,19 @@ def waveform(rng, n=600):
 
 
 def random_probes(rng, shape, count):
+    """Random fields and cells; every other probe is an E component on a
+    boundary face, where the Mur faces write in the probes' own launch."""
     nx, ny, nz = shape
-    return np.stack([rng.integers(0, 9, count), rng.integers(0, nx, count), rng.integers(0, ny, count),
-                     rng.integers(0, nz, count)], 1).astype(np.int32)
+    p = np.stack([rng.integers(0, 9, count), rng.integers(0, nx, count), rng.integers(0, ny, count),
+                  rng.integers(0, nz, count)], 1).astype(np.int32)
+    for r in range(1, count, 2):
+        p[r, 0] = 6 + rng.integers(0, 3)
+        ax = int(rng.integers(0, 3))
+        p[r, 1 + ax] = 0 if rng.integers(0, 2) == 0 else shape[ax] - 1
+    return p
 
 
-def make_case(seed, shape, nmat, steps, storage=F.STORE_F32, jz_mode=F.JZ_STEP, rp=False, nprobe=3,
+def make_case(seed, shape, nmat, steps, storage=F.STORE_F32, jz_mode=F.JZ_STEP, rp=False, nprobe=6,
               debye=F.DEBYE_FIXED, jz_limit=0):
     rng = np.random.default_rng(seed)
     ids, table = random_materials(rng, shape, nmat)

@@ -243,3 +243,48 @@ def test_probes_match_fields_each_step():
         row = ctx.probes()[n]
         for p, (f, i, j, k) in enumerate(probes):
             assert row[p] == np.float64(ctx.get(f)[k, j, i])
+
+
+def _face_probes(shape):
+    """One cell of each of the 12 Mur segments, E cells on faces, edges and
+    corners that no segment updates, and D/H cells on faces."""
+    nx, ny, nz = shape
+    i, j, k = nx // 2, ny // 3, nz // 2
+    cells = [
+        (6, i, 0, k), (6, i, ny - 1, k), (6, i, j, 0), (6, i, j, nz - 1),          # Ex segments 0-3
+        (7, 0, j, k), (7, nx - 1, j, k), (7, i, j, 0), (7, i, j, nz - 1),          # Ey segments 4-7
+        (8, 0, j, k), (8, nx - 1, j, k), (8, i, 0, k), (8, i, ny - 1, k),          # Ez segments 8-11
+        (6, 0, 0, 0), (6, nx - 1, 0, k), (6, i, 0, 0), (7, 0, ny - 1, k), (7, 0, j, 0),
+        (8, 0, 0, k), (8, i, j, nz - 1), (8, nx - 1, ny - 1, nz - 1), (7, nx - 1, ny - 2, nz - 2),
+        (2, 0, j, k), (3, i, 0, k), (5, i, j, nz - 1), (1, nx - 1, j, 0),
+    ]
+    return np.asarray(cells, np.int32)
+
+
+@pytest.mark.parametrize("shape,kchunk,tile_x", [((40, 21, 19), 4, 64),    # x faces in the fused kernel
+                                                 ((65, 12, 11), 4, 64),    # x-high faces in the Mur kernel
+                                                 ((65, 12, 11), 3, 128),
+                                                 ((40, 21, 19), 20, 64)])  # all faces in the Mur kernel
+def test_face_probes_match_fields(shape, kchunk, tile_x):
+    """Probe rows are recorded in the Mur launch; on an E face cell that
+    launch updates, the probe evaluates the face itself.  Every row equals
+    the fields read back after the step."""
+    print(f"face probes {shape} kc={kchunk} tx={tile_x}", flush=True)
+    rng = np.random.default_rng(11)
+    ids, table = random_materials(rng, shape)
+    probes = _face_probes(shape)
+    ctx = F.Context(_params(shape), tile_y=4, kchunk=kchunk, tile_x=tile_x)
+    ctx.set_materials(ids, table)
+    ctx.set_rp(random_rp(rng, shape))
+    ctx.set_sources(random_sources(rng, shape, 2), rng.uniform(-1, 1, 10).astype(np.float32))
+    for (f, lv), a in random_state(rng, shape).items():
+        ctx.set(f, lv, a)
+    steps = 6
+    ctx.set_probes(probes, steps)
+    for n in range(steps):
+        ctx.step()
+        row = ctx.probes()[n]
+        for p, (f, i, j, k) in enumerate(probes):
+            want = ctx.get(f)[k, j, i]
+            assert want != 0.0 or p >= 12, f"probe {p}: segment face value is zero"
+            assert row[p] == np.float64(want), f"step {n + 1} probe {p} {O.FIELDS[f]} ({i},{j},{k})"

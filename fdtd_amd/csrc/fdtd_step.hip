@@ -520,6 +520,16 @@ __device__ __forceinline__ void stw_nt(void *p, uint32_t b, uint32_t w)
 // fused_tile<TY, T, false>, operation for operation, so the two kernels give
 // identical results.  Pairs are split into 16-bit stores only where the two
 // cells' update ranges differ (the x boundaries and the padding).
+// LDS row of a paired tile, split into even and odd cells: column c of the
+// unsplit row (cell x = c - 1 of the tile, x in [-1, 2 * TX]) is half x & 1,
+// index (x >> 1) + 1.
+template <int N>
+__device__ __forceinline__ float &lx(float (&row)[2][N], int c)
+{
+    const int x = c - 1;
+    return row[x & 1][(x >> 1) + 1];
+}
+
 template <int TY, typename T>
 __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int by, int bz)
 {
@@ -529,8 +539,11 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
     constexpr uint32_t ES = 2;
     using X = Elem<T>;
     using P = Pair<T>;
-    __shared__ float sE[2][3][TY + 2][TC + 2];
-    __shared__ float sH[3][TY + 1][TC + 1];
+    // Rows hold the even cells (x = 2m) and the odd cells (x = 2m + 1) in two
+    // halves (lx), so the two cells of lane tx sit at index tx + 1 of each
+    // half and every lane-indexed access is unit-stride: no bank conflicts.
+    __shared__ float sE[2][3][TY + 2][2][TX + 2];
+    __shared__ float sH[3][TY + 1][2][TX + 1];
     __shared__ float sJunk[2][TX];
     __shared__ float4 sMurC[2][kMurK][TY];
     __shared__ float sMurT[2][TY];
@@ -590,19 +603,19 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
     };
     auto stage_e = [&](int buf) {
         for (int c = 0; c < 3; c++) {
-            sE[buf][c][ty + 1][ca] = P::lo(pe[c]);
-            sE[buf][c][ty + 1][ca + 1] = P::hi(pe[c]);
+            lx(sE[buf][c][ty + 1], ca) = P::lo(pe[c]);
+            lx(sE[buf][c][ty + 1], ca + 1) = P::hi(pe[c]);
         }
         if (w_rlo)
             for (int c = 0; c < 3; c++) {
-                sE[buf][c][0][ca] = P::lo(pr[c]);
-                sE[buf][c][0][ca + 1] = P::hi(pr[c]);
+                lx(sE[buf][c][0], ca) = P::lo(pr[c]);
+                lx(sE[buf][c][0], ca + 1) = P::hi(pr[c]);
             }
         if (w_rhi) {
-            sE[buf][0][TY + 1][ca] = P::lo(pq[0]); sE[buf][0][TY + 1][ca + 1] = P::hi(pq[0]);
-            sE[buf][2][TY + 1][ca] = P::lo(pq[1]); sE[buf][2][TY + 1][ca + 1] = P::hi(pq[1]);
+            lx(sE[buf][0][TY + 1], ca) = P::lo(pq[0]); lx(sE[buf][0][TY + 1], ca + 1) = P::hi(pq[0]);
+            lx(sE[buf][2][TY + 1], ca) = P::lo(pq[1]); lx(sE[buf][2][TY + 1], ca + 1) = P::hi(pq[1]);
         }
-        if (w_col) { sE[buf][0][cr][cl] = pc[0]; sE[buf][1][cr][cl] = pc[1]; sE[buf][2][cr][cl] = pc[2]; }
+        if (w_col) { lx(sE[buf][0][cr], cl) = pc[0]; lx(sE[buf][1][cr], cl) = pc[1]; lx(sE[buf][2][cr], cl) = pc[2]; }
     };
 
     // Own-pair operands of one plane (words), two halo words, the id pair.
@@ -660,10 +673,10 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
         for (int h = 0; h < 2; h++) {
             const int c = ca + h, i = ia + h;
             const float hxv = half_of(hx0, h), hyv = half_of(hy0, h);
-            const float exk = sE[0][0][ty + 1][c], eyk = sE[0][1][ty + 1][c], ezk = sE[0][2][ty + 1][c];
-            const float exk1 = sE[1][0][ty + 1][c], eyk1 = sE[1][1][ty + 1][c];
-            const float ux = hxv - chy * (sE[0][2][ty + 2][c] - ezk) + chy * (eyk1 - eyk);
-            const float uy = hyv - chz * (exk1 - exk) + chx * (sE[0][2][ty + 1][c + 1] - ezk);
+            const float exk = lx(sE[0][0][ty + 1], c), eyk = lx(sE[0][1][ty + 1], c), ezk = lx(sE[0][2][ty + 1], c);
+            const float exk1 = lx(sE[1][0][ty + 1], c), eyk1 = lx(sE[1][1][ty + 1], c);
+            const float ux = hxv - chy * (lx(sE[0][2][ty + 2], c) - ezk) + chy * (eyk1 - eyk);
+            const float uy = hyv - chz * (exk1 - exk) + chx * (lx(sE[0][2][ty + 1], c + 1) - ezk);
             hxm[h] = (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1) ? ux : hxv;
             hym[h] = (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1) ? uy : hyv;
         }
@@ -683,8 +696,8 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
     const bool col_ok = (side == 0) && (cr >= 1) && (cr <= TY) && (i0 >= 1);
     const bool cy_ij = col_ok && (i0 - 1 < nx - 1) && (jc >= 1 && jc < ny - 1);
     const bool cz_ij = col_ok && (i0 - 1 < nx - 1) && (jc < ny - 1);
-    float *col_dst_y = col_ok ? &sH[1][ccr][0] : &sJunk[0][tx];
-    float *col_dst_z = col_ok ? &sH[2][ccr][0] : &sJunk[1][tx];
+    float *col_dst_y = col_ok ? &lx(sH[1][ccr], 0) : &sJunk[0][tx];
+    float *col_dst_z = col_ok ? &lx(sH[2][ccr], 0) : &sJunk[1][tx];
 
     const float jzv = A.jz;
     for (int k = k0; k < k1; ++k) {
@@ -702,38 +715,38 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int c = ca + h, i = ia + h;
-            exk[h] = sE[cur][0][ty + 1][c]; eyk[h] = sE[cur][1][ty + 1][c]; ezk[h] = sE[cur][2][ty + 1][c];
-            const float exk1 = sE[nxt][0][ty + 1][c], eyk1 = sE[nxt][1][ty + 1][c];
+            exk[h] = lx(sE[cur][0][ty + 1], c); eyk[h] = lx(sE[cur][1][ty + 1], c); ezk[h] = lx(sE[cur][2][ty + 1], c);
+            const float exk1 = lx(sE[nxt][0][ty + 1], c), eyk1 = lx(sE[nxt][1][ty + 1], c);
             const float hxo_ = half_of(cu.hxo, h), hyo_ = half_of(cu.hyo, h), hzo_ = half_of(cu.hzo, h);
-            const float ux = hxo_ - chy * (sE[cur][2][ty + 2][c] - ezk[h]) + chy * (eyk1 - eyk[h]);
-            const float uy = hyo_ - chz * (exk1 - exk[h]) + chx * (sE[cur][2][ty + 1][c + 1] - ezk[h]);
-            const float uz = hzo_ - chx * (sE[cur][1][ty + 1][c + 1] - eyk[h]) + chy * (sE[cur][0][ty + 2][c] - exk[h]);
+            const float ux = hxo_ - chy * (lx(sE[cur][2][ty + 2], c) - ezk[h]) + chy * (eyk1 - eyk[h]);
+            const float uy = hyo_ - chz * (exk1 - exk[h]) + chx * (lx(sE[cur][2][ty + 1], c + 1) - ezk[h]);
+            const float uz = hzo_ - chx * (lx(sE[cur][1][ty + 1], c + 1) - eyk[h]) + chy * (lx(sE[cur][0][ty + 2], c) - exk[h]);
             hx[h] = (i >= 1 && i < nx - 1 && j < ny - 1 && kh) ? ux : hxo_;
             hy[h] = (i < nx - 1 && j >= 1 && j < ny - 1 && kh) ? uy : hyo_;
             hz[h] = (i < nx - 1 && j < ny - 1 && kz) ? uz : hzo_;
-            sH[0][ty + 1][c] = hx[h];
-            sH[1][ty + 1][c] = hy[h];
-            sH[2][ty + 1][c] = hz[h];
+            lx(sH[0][ty + 1], c) = hx[h];
+            lx(sH[1][ty + 1], c) = hy[h];
+            lx(sH[2][ty + 1], c) = hz[h];
         }
         if (w_rlo) {  // row pair (ia + h, j0-1): Hx, Hz
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int c = ca + h, i = ia + h;
                 const float h0 = half_of(cu.hh0, h), h1 = half_of(cu.hh1, h);
-                const float ezh = sE[cur][2][0][c], eyh = sE[cur][1][0][c], exh = sE[cur][0][0][c];
-                const float vx = h0 - chy * (sE[cur][2][1][c] - ezh) + chy * (sE[nxt][1][0][c] - eyh);
-                const float vz = h1 - chx * (sE[cur][1][0][c + 1] - eyh) + chy * (sE[cur][0][1][c] - exh);
+                const float ezh = lx(sE[cur][2][0], c), eyh = lx(sE[cur][1][0], c), exh = lx(sE[cur][0][0], c);
+                const float vx = h0 - chy * (lx(sE[cur][2][1], c) - ezh) + chy * (lx(sE[nxt][1][0], c) - eyh);
+                const float vz = h1 - chx * (lx(sE[cur][1][0], c + 1) - eyh) + chy * (lx(sE[cur][0][1], c) - exh);
                 const bool rx = (jh >= 0) && (i >= 1 && i < nx - 1 && jh < ny - 1);
                 const bool rz = (jh >= 0) && (i < nx - 1 && jh < ny - 1);
-                sH[0][0][c] = (rx && kh) ? vx : h0;
-                sH[2][0][c] = (rz && kz) ? vz : h1;
+                lx(sH[0][0], c) = (rx && kh) ? vx : h0;
+                lx(sH[2][0], c) = (rz && kz) ? vz : h1;
             }
         }
         if (w_col) {  // (i0-1, j0-1+ccr): Hy, Hz
             const float h0 = __uint_as_float(cu.hh0), h1 = __uint_as_float(cu.hh1);
-            const float exh = sE[cur][0][ccr][0], eyh = sE[cur][1][ccr][0], ezh = sE[cur][2][ccr][0];
-            const float vy = h0 - chz * (sE[nxt][0][ccr][0] - exh) + chx * (sE[cur][2][ccr][1] - ezh);
-            const float vz = h1 - chx * (sE[cur][1][ccr][1] - eyh) + chy * (sE[cur][0][ccr + 1][0] - exh);
+            const float exh = lx(sE[cur][0][ccr], 0), eyh = lx(sE[cur][1][ccr], 0), ezh = lx(sE[cur][2][ccr], 0);
+            const float vy = h0 - chz * (lx(sE[nxt][0][ccr], 0) - exh) + chx * (lx(sE[cur][2][ccr], 1) - ezh);
+            const float vz = h1 - chx * (lx(sE[cur][1][ccr], 1) - eyh) + chy * (lx(sE[cur][0][ccr + 1], 0) - exh);
             *col_dst_y = (cy_ij && kh) ? vy : h0;
             *col_dst_z = (cz_ij && kz) ? vz : h1;
         }
@@ -760,9 +773,9 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
             rdz[h] = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) && kh;
             reyz[h] = (i >= 1) && (k >= 1);
             const float dxn_ = half_of(cu.dxn, h), dyn_ = half_of(cu.dyn, h), dzn_ = half_of(cu.dzn, h);
-            dxu[h] = dxn_ + cdy * (hz[h] - sH[2][ty][c]) - cdz * (hy[h] - hym[h]);
-            dyu[h] = dyn_ + cdz * (hx[h] - hxm[h]) - cdx * (hz[h] - sH[2][ty + 1][c - 1]);
-            float dz_ = dzn_ + cdx * (hy[h] - sH[1][ty + 1][c - 1]) - cdy * (hx[h] - sH[0][ty][c]);
+            dxu[h] = dxn_ + cdy * (hz[h] - lx(sH[2][ty], c)) - cdz * (hy[h] - hym[h]);
+            dyu[h] = dyn_ + cdz * (hx[h] - hxm[h]) - cdx * (hz[h] - lx(sH[2][ty + 1], c - 1));
+            float dz_ = dzn_ + cdx * (hy[h] - lx(sH[1][ty + 1], c - 1)) - cdy * (hx[h] - lx(sH[0][ty], c));
             const uint32_t id = (cu.id >> (8 * h)) & 0xffu;
             dz_ = (id & kSourceBit) ? dz_ - jzv : dz_;
             dzu[h] = dz_;
@@ -779,12 +792,12 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
         if (mur_lo && tx == 0 && j < ny) {  // face i = 0 is cell a of lane 0, its neighbour cell b
             const float4 cm = sMurC[0][k - k0][ty];
             if (j < ny - 1 && k >= 1 && k < nz - 1) {
-                eyn[0] = cm.x * ((cm.y * X::rnd(eyn[1]) + cm.z * sE[cur][1][ty + 1][2]) - cm.w * eyk[0]);
+                eyn[0] = cm.x * ((cm.y * X::rnd(eyn[1]) + cm.z * lx(sE[cur][1][ty + 1], 2)) - cm.w * eyk[0]);
                 sty[0] = true;
             }
             if (j >= 1 && j < ny - 1 && k < nz - 1) {
                 const float tn = k >= 1 ? X::rnd(ezn[1]) : sMurT[0][ty];
-                ezn[0] = cm.x * ((cm.y * tn + cm.z * sE[cur][2][ty + 1][2]) - cm.w * ezk[0]);
+                ezn[0] = cm.x * ((cm.y * tn + cm.z * lx(sE[cur][2][ty + 1], 2)) - cm.w * ezk[0]);
                 stz[0] = true;
             }
         }
@@ -799,13 +812,13 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
                 const float4 cm = sMurC[1][k - k0][ty];
                 const int c = ca + h;
                 if (j < ny - 1 && k >= 1 && k < nz - 1) {
-                    const float sn = sE[cur][1][ty + 1][c - 1];
+                    const float sn = lx(sE[cur][1][ty + 1], c - 1);
                     eyn[h] = cm.x * ((cm.y * sn + cm.z * sn) - cm.w * eyk[h]);
                     sty[h] = true;
                 }
                 if (j >= 1 && j < ny - 1 && k < nz - 1) {
                     const float tn = k >= 1 ? (h ? X::rnd(ezn[0]) : ezm) : sMurT[1][ty];
-                    ezn[h] = cm.x * ((cm.y * tn + cm.z * sE[cur][2][ty + 1][c - 1]) - cm.w * ezk[h]);
+                    ezn[h] = cm.x * ((cm.y * tn + cm.z * lx(sE[cur][2][ty + 1], c - 1)) - cm.w * ezk[h]);
                     stz[h] = true;
                 }
             }
@@ -823,9 +836,8 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
     }
 }
 
-// One grid, one step: block b runs tile xcd_logical(b).  Five waves per
-// SIMD fit in 96 VGPRs without spilling (the persistent batch kernel would
-// spill, so it keeps four).
+// One grid, one step: block b runs tile xcd_logical(b), at five waves per
+// SIMD (96 VGPRs).
 template <int TY, typename T, bool EXACT, int TXW>
 __global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) void fused_step(const StepArgs A)
 {
@@ -896,9 +908,12 @@ __device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, Step
 // (instances in list order, each in z-chunk-major tile order).  Tiles of
 // instances outside their step window are skipped.
 // The paired 16-bit form fits four waves per SIMD in 128 VGPRs without
-// spilling (left alone it takes 142 and three).
+// spilling (left alone it takes 142 and three).  The scaled-unit one-cell
+// forms run five waves per SIMD in 96 VGPRs with a small spill (32-60 bytes
+// per lane), measured 5% faster on 64 uneven fp16 grids than four waves;
+// the exact fp32 form gains nothing from it and keeps four without spilling.
 template <int TY, typename T, bool EXACT, bool PAIR>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4 : 1))) void fused_batch(
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4 : (EXACT ? 1 : 5)))) void fused_batch(
     const BatchLaunch L)
 {
     const int tid = threadIdx.y * TX + threadIdx.x;

@@ -283,7 +283,7 @@ def bench_batched(a):
                    "graph_steps": a.segment, "persistent_grid": info["grid"][0], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(97 * work / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(97 * work / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "whole batch step (fused_batch + mur_batch + probe_batch)",
+                     "traffic": None, "kernel": "whole batch step (fused_batch + mur_batch with probe rows)",
                      "bytes_per_cell": 97},
         "upload_s": round(t_upload, 3), "all_finished": ok,
     }

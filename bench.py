@@ -2,8 +2,8 @@
 """Throughput of the fused FDTD time step on one MI355X.
 
 A "step" is one full FDTD time step of the grid: the fused H/D/source/Debye-E
-kernel, the Mur face kernel, the probe kernel and (every 100 steps) the
-z-slice capture.  The default workload is the 256^3 eight-ellipsoid Debye
+kernel, the Mur face kernel (which also records the probe row) and (every
+100 steps) the z-slice capture.  The default workload is the 256^3 eight-ellipsoid Debye
 phantom with a Jz sheet source at 900 MHz, 3 probes and a z-slice every 100
 steps (plain_256 in fdtd_amd/workloads.py), 2000 steps.
 
@@ -26,6 +26,11 @@ Two more workloads measure the systems patterns:
                       p50/p99 latency beside the sequential drop-in path (the
                       five reference entry points, one solve at a time, FIFO)
                       on the same stream.
+  --workload slabs    320x320x880 cut into --slabs z-slabs (each with ghost
+                      planes), all stepped on this one GPU with the halo
+                      exchange of every interface before every step: the
+                      part of a multi-GPU z-slab run one GPU can show, beside
+                      the unsplit grid.
 """
 import argparse
 import json
@@ -51,7 +56,8 @@ def parse():
                     help="HIP events around every N-th fused launch of the timed steps (each pair costs the "
                          "stream a few microseconds)")
     ap.add_argument("--workload", default="plain_256",
-                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken"])
+                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken", "slabs"])
+    ap.add_argument("--slabs", type=int, default=8, help="slabs: z-slabs of the 320x320x880 grid")
     ap.add_argument("--phantom", default=os.path.join(ROOT, "data", "riken_ids.npz"),
                     help="riken: PGM slice directory or ids packed by `python -m fdtd_amd.phantom`")
     ap.add_argument("--specs", default=os.path.join(ROOT, "tests", "golden", "mat_specs_riken"),
@@ -126,6 +132,8 @@ def main():
         return bench_batched(a)
     if a.workload == "served":
         return bench_served(a)
+    if a.workload == "slabs":
+        return bench_slabs(a)
 
     total = a.warmup + a.steps
     w = build_workload(a.workload, total, a)
@@ -196,6 +204,103 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps)
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_slabs(a):
+    """large_phantom as `a.slabs` z-slabs on one GPU, halo exchange every step."""
+    import fdtd_amd as F
+    from fdtd_amd import hip
+    from fdtd_amd import workloads as W
+    total = a.warmup + a.steps
+    w = W.large_phantom(steps=total)
+    nx, ny, nz = w.shape
+    p = w.params
+    n = max(2, a.slabs)
+    bounds = [nz * s // n for s in range(n + 1)]
+    stream = hip.Stream()
+    s_ = stream.handle
+    slabs = []
+    for s in range(n):
+        r0, r1 = bounds[s], bounds[s + 1]
+        g0, g1 = r0 - (s > 0), r1 + (s < n - 1)
+        c = F.Context(F.make_params((nx, ny, g1 - g0), p.dt, p.dx, mu0=p.mu0, eps0=p.eps0))
+        c.set_materials(w.ids[g0:g1], w.table)
+        own = w.sources[(w.sources[:, 2] >= r0) & (w.sources[:, 2] < r1)].copy()
+        own[:, 2] -= g0
+        c.set_sources(own, w.jz)
+        slabs.append(c)
+    up_b, down_b = slabs[0].halo_bytes(F.HALO_UP), slabs[0].halo_bytes(F.HALO_DOWN)
+    ups = [hip.DeviceArray(up_b // 4) for _ in range(n - 1)]
+    downs = [hip.DeviceArray(down_b // 4) for _ in range(n - 1)]
+
+    def exchange():
+        for q in range(n - 1):
+            lo, hi = slabs[q], slabs[q + 1]
+            lo.halo_pack(F.HALO_UP, ups[q].ptr, s_)
+            hi.halo_unpack(F.HALO_UP, ups[q].ptr, s_)
+            hi.halo_pack(F.HALO_DOWN, downs[q].ptr, s_)
+            lo.halo_unpack(F.HALO_DOWN, downs[q].ptr, s_)
+
+    def step_all(k):
+        for _ in range(k):
+            exchange()
+            for c in slabs:
+                c.step(s_)
+
+    step_all(a.warmup)
+    stream.sync()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    step_all(a.steps)
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1)
+    # the exchange alone, same number of steps (the fields are left as they are)
+    x0, x1 = hip.Event(), hip.Event()
+    x0.record(stream)
+    for _ in range(a.steps):
+        exchange()
+    x1.record(stream)
+    x1.sync()
+    xms = x0.elapsed_ms(x1)
+    for c in slabs:
+        c.close()
+    for b in ups + downs:
+        b.free()
+    # the unsplit grid, same steps
+    whole = F.Context(p)
+    whole.set_materials(w.ids, w.table)
+    whole.set_sources(w.sources, w.jz)
+    whole.run(a.warmup, s_)
+    stream.sync()
+    u0, u1 = hip.Event(), hip.Event()
+    u0.record(stream)
+    whole.run(a.steps, s_)
+    u1.record(stream)
+    u1.sync()
+    ums = u0.elapsed_ms(u1)
+    whole.close()
+    cells = nx * ny * nz
+    out = {
+        "metric": f"Gcell-updates/s ({nx}x{ny}x{nz} as {n} z-slabs on one GPU, halo exchange every step)",
+        "value": round(cells * a.steps / (ms * 1e-3) / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms / a.steps, 5),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 Debye sums)",
+        "data": "synthetic (seeded phantom)",
+        "config": {"workload": "large_phantom_slabs", "grid": [nx, ny, nz], "slabs": n,
+                   "slab_planes": [bounds[s + 1] - bounds[s] for s in range(n)], "parallelism": "single GPU"},
+        "exchange_us_per_step": round(xms / a.steps * 1e3, 2),
+        "message_bytes_per_interface": {"up": up_b, "down": down_b},
+        "unsplit_value": round(cells * a.steps / (ums * 1e-3) / 1e9, 3),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -48,7 +48,7 @@ EXPORTS = {
                    "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
-                   "fdtd_ctx_autotune",
+                   "fdtd_ctx_autotune", "fdtd_ctx_halo_bytes", "fdtd_ctx_halo_pack", "fdtd_ctx_halo_unpack",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
@@ -58,6 +58,7 @@ EXPORTS = {
 }
 ENOMEM = -5
 PGM_REFERENCE, PGM_IMAGE = 0, 1
+HALO_UP, HALO_DOWN = 0, 1
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
@@ -174,6 +175,9 @@ def lib():
         L.fdtd_ctx_load.argtypes = [vp, C.c_char_p]
         L.fdtd_energy_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.fdtd_ctx_halo_bytes.argtypes = [vp, i, C.POINTER(C.c_size_t)]
+        L.fdtd_ctx_halo_pack.argtypes = [vp, i, vp, vp]
+        L.fdtd_ctx_halo_unpack.argtypes = [vp, i, vp, vp]
         PP, PE, PI = C.POINTER(FdtdParams), C.POINTER(FdtdExt), C.POINTER(FdtdInstance)
         L.fdtd_batch_create.argtypes = [C.c_size_t, C.POINTER(vp)]
         L.fdtd_batch_destroy.argtypes = [vp]
@@ -388,6 +392,20 @@ class Context:
         p, pl, e = C.c_int(), C.c_longlong(), C.c_int()
         check(lib().fdtd_ctx_layout(self._h, C.byref(p), C.byref(pl), C.byref(e)), "layout")
         return p.value, pl.value, e.value
+
+    def halo_bytes(self, direction):
+        """Bytes of a z-slab halo message (HALO_UP or HALO_DOWN)."""
+        n = C.c_size_t()
+        check(lib().fdtd_ctx_halo_bytes(self._h, direction, C.byref(n)), "halo_bytes")
+        return n.value
+
+    def halo_pack(self, direction, dst, stream=None):
+        """Pack this slab's outgoing halo planes into device buffer `dst`."""
+        check(lib().fdtd_ctx_halo_pack(self._h, direction, dst, stream), "halo_pack")
+
+    def halo_unpack(self, direction, src, stream=None):
+        """Unpack a neighbour's halo message (device buffer) into a ghost plane."""
+        check(lib().fdtd_ctx_halo_unpack(self._h, direction, src, stream), "halo_unpack")
 
     def launch_shape(self):
         v = [C.c_int() for _ in range(4)]

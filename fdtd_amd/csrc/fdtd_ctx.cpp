@@ -624,6 +624,74 @@ extern "C" int fdtd_ctx_layout(fdtd_ctx *c, int *pitch, long long *plane, int *e
     return 0;
 }
 
+namespace {
+
+// The planes of one halo message: FDTD_HALO_UP carries E^n (x, y, z) and
+// H^{n-1/2} (x, y) of the lower slab's last real plane into the upper
+// slab's ghost plane 0; FDTD_HALO_DOWN carries E^n (x, y) of the upper
+// slab's first real plane into the lower slab's ghost plane nz-1.
+int halo_planes(const fdtd_ctx *c, int dir, bool sending, PlaneCopy &pc, char **fields, long long *plane_index)
+{
+    static const int up[] = {FDTD_EX, FDTD_EY, FDTD_EZ, FDTD_HX, FDTD_HY};
+    static const int down[] = {FDTD_EX, FDTD_EY};
+    if (dir != FDTD_HALO_UP && dir != FDTD_HALO_DOWN) return fdtd_fail(FDTD_EINVAL, "halo direction %d", dir);
+    if (c->nz < 3) return fdtd_fail(FDTD_ERANGE, "a slab needs a ghost plane and a real plane");
+    const int *f = dir == FDTD_HALO_UP ? up : down;
+    pc.n = dir == FDTD_HALO_UP ? 5 : 2;
+    pc.words = c->plane * (long long)c->esz / 16;
+    *plane_index = dir == FDTD_HALO_UP ? (sending ? c->nz - 2 : 0) : (sending ? 1 : c->nz - 1);
+    for (int q = 0; q < pc.n; q++) fields[q] = field_dev(const_cast<fdtd_ctx *>(c), f[q], 0);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int fdtd_ctx_halo_bytes(fdtd_ctx *c, int dir, size_t *bytes)
+{
+    if (!c || !bytes) return fdtd_fail(FDTD_EINVAL, "null argument");
+    PlaneCopy pc{};
+    char *fields[kHaloFields];
+    long long k;
+    if (int rc = halo_planes(c, dir, true, pc, fields, &k)) return rc;
+    *bytes = (size_t)pc.n * (size_t)pc.words * 16;
+    return 0;
+}
+
+extern "C" int fdtd_ctx_halo_pack(fdtd_ctx *c, int dir, void *dst, hipStream_t stream)
+{
+    if (!c || !dst) return fdtd_fail(FDTD_EINVAL, "null argument");
+    if ((uintptr_t)dst % 16) return fdtd_fail(FDTD_EINVAL, "halo buffer must be 16-byte aligned");
+    PlaneCopy pc{};
+    char *fields[kHaloFields];
+    long long k;
+    if (int rc = halo_planes(c, dir, true, pc, fields, &k)) return rc;
+    const size_t pb = (size_t)pc.words * 16;
+    for (int q = 0; q < pc.n; q++) {
+        pc.src[q] = (const uint4 *)(fields[q] + (size_t)k * pb);
+        pc.dst[q] = (uint4 *)((char *)dst + q * pb);
+    }
+    const hipError_t e = launch_plane_copy(pc, stream);
+    return e == hipSuccess ? 0 : fdtd_hip_fail(e, "halo pack");
+}
+
+extern "C" int fdtd_ctx_halo_unpack(fdtd_ctx *c, int dir, const void *src, hipStream_t stream)
+{
+    if (!c || !src) return fdtd_fail(FDTD_EINVAL, "null argument");
+    if ((uintptr_t)src % 16) return fdtd_fail(FDTD_EINVAL, "halo buffer must be 16-byte aligned");
+    PlaneCopy pc{};
+    char *fields[kHaloFields];
+    long long k;
+    if (int rc = halo_planes(c, dir, false, pc, fields, &k)) return rc;
+    const size_t pb = (size_t)pc.words * 16;
+    for (int q = 0; q < pc.n; q++) {
+        pc.src[q] = (const uint4 *)((const char *)src + q * pb);
+        pc.dst[q] = (uint4 *)(fields[q] + (size_t)k * pb);
+    }
+    c->fields_set = true;
+    const hipError_t e = launch_plane_copy(pc, stream);
+    return e == hipSuccess ? 0 : fdtd_hip_fail(e, "halo unpack");
+}
+
 extern "C" int fdtd_ctx_set_energy(fdtd_ctx *c, int interval, int capacity)
 {
     if (!c || interval < 0 || capacity < 0) return fdtd_fail(FDTD_EINVAL, "bad energy series");

@@ -155,6 +155,15 @@ hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
+// Halo planes of a z-slab: n runs of `words` 16-byte words each.
+constexpr int kHaloFields = 5;
+struct PlaneCopy {
+    const uint4 *src[kHaloFields];
+    uint4 *dst[kHaloFields];
+    int n;
+    long long words;
+};
+hipError_t launch_plane_copy(const PlaneCopy &p, hipStream_t s);
 // Mur faces and one probe row in one launch (p.nprobe = 0: faces only).  The
 // probe fields must be the newest levels, with E the Mur target a.exT..ezT.
 hipError_t launch_mur_probes(const MurArgs &a, const ProbeArgs &p, int storage, hipStream_t s);

@@ -1232,6 +1232,32 @@ hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s)
     return hipGetLastError();
 }
 
+namespace {
+
+// Copies up to kHaloFields equal-sized runs of 16-byte words, one run per
+// blockIdx.y (halo planes into a message buffer and back).
+__global__ __launch_bounds__(256) void plane_copy(const PlaneCopy P)
+{
+    const int f = (int)blockIdx.y;
+    const uint4 *src = P.src[0];
+    uint4 *dst = P.dst[0];
+#pragma unroll
+    for (int q = 1; q < kHaloFields; q++)
+        if (f == q) { src = P.src[q]; dst = P.dst[q]; }
+    for (long long w = (long long)blockIdx.x * 256 + threadIdx.x; w < P.words; w += (long long)gridDim.x * 256)
+        dst[w] = src[w];
+}
+
+}  // namespace
+
+hipError_t launch_plane_copy(const PlaneCopy &p, hipStream_t s)
+{
+    if (p.n <= 0 || p.n > kHaloFields || p.words <= 0) return hipErrorInvalidValue;
+    const long long blocks = std::min<long long>((p.words + 255) / 256, 1024);
+    plane_copy<<<dim3((unsigned)blocks, (unsigned)p.n), 256, 0, s>>>(p);
+    return hipGetLastError();
+}
+
 hipError_t launch_mur_probes(const MurArgs &a, const ProbeArgs &p, int storage, hipStream_t s)
 {
     const int mb = (a.seg_start[12] + kMurBlock - 1) / kMurBlock;

@@ -124,6 +124,30 @@ int fdtd_ctx_field_ptr(fdtd_ctx *ctx, int field, int level, void **dev);
  * k*plane + j*pitch + i; pitch >= nx rounds rows up to 128-byte lines,
  * plane = pitch*ny; elem_bytes is 4 or 2. */
 int fdtd_ctx_layout(fdtd_ctx *ctx, int *pitch, long long *plane, int *elem_bytes);
+
+/* z-slab decomposition (no counterpart in the reference, which runs one
+ * grid on one device).  A context may hold one z-slab of a larger grid plus
+ * a ghost plane on each side that has a neighbouring slab: local plane 0
+ * below, local plane nz-1 above.  Before every step each interface
+ * exchanges one message each way, and then both slabs step; their real
+ * planes then equal the whole grid's bit for bit (same storage form and
+ * scales, sources and probes given to the slab that owns the cell, the
+ * global grid's rp planes sliced to the slab).
+ *   FDTD_HALO_UP:   packed by the lower slab from its last real plane
+ *                   (E^n x, y, z and H^{n-1/2} x, y), unpacked by the upper
+ *                   slab into its ghost plane 0;
+ *   FDTD_HALO_DOWN: packed by the upper slab from its first real plane
+ *                   (E^n x, y), unpacked by the lower slab into its ghost
+ *                   plane nz-1.
+ * A message is a contiguous device buffer of fdtd_ctx_halo_bytes bytes
+ * (whole pitched planes, one per field), 16-byte aligned: what a
+ * point-to-point send/recv between two GPUs carries.  Pack and unpack are
+ * one kernel launch each on `stream`. */
+#define FDTD_HALO_UP 0
+#define FDTD_HALO_DOWN 1
+int fdtd_ctx_halo_bytes(fdtd_ctx *ctx, int dir, size_t *bytes);
+int fdtd_ctx_halo_pack(fdtd_ctx *ctx, int dir, void *dst, hipStream_t stream);
+int fdtd_ctx_halo_unpack(fdtd_ctx *ctx, int dir, const void *src, hipStream_t stream);
 /* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales are 1/32 of
  * the largest per-step D/eps0 kick of the source waveform, fixed at the
  * first step. */

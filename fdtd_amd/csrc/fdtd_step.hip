@@ -211,7 +211,10 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     __shared__ float sH[3][TY + 1][TXW + 1];
     __shared__ float sJunk[2][TXW];
 
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // every wave covers one tile row (64 lanes along x), so the row index is
+    // wave-uniform: as a scalar it keeps the row tests and the wave roles
+    // (halo rows, column gather) out of the per-lane masks
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int i0 = bx * TXW, j0 = by * TY;
     const int k0 = bz * A.kchunk;
@@ -548,7 +551,10 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
     __shared__ float4 sMurC[2][kMurK][TY];
     __shared__ float sMurT[2][TY];
 
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // every wave covers one tile row (64 lanes along x), so the row index is
+    // wave-uniform: as a scalar it keeps the row tests and the wave roles
+    // (halo rows, column gather) out of the per-lane masks
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int i0 = bx * TC, j0 = by * TY;
     const int k0 = bz * A.kchunk;

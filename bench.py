@@ -93,6 +93,28 @@ def build_workload(name, steps, a=None):
     return W.plain_small(steps=steps)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "fused_step_plain256_large_phantom.md")
+
+
+def profiled_traffic(workload, storage):
+    """HBM bytes per fused launch measured for this workload (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB copy) in the committed
+    profile, or None."""
+    if storage != "f32" or workload not in ("plain_256", "large_phantom") or not os.path.exists(PROFILE):
+        return None
+    import re
+    text = open(PROFILE).read()
+    head = f"## {workload}:"
+    if head not in text:
+        return None
+    sec = text.split(head, 1)[1].split("\n## ", 1)[0]
+    rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
+    wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
+    if not rd or not wr:
+        return None
+    return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024
+
+
 def cpu_baseline(w, steps):
     """The oracle's FP32 transcription on this host's cores (a baseline only)."""
     from oracle import oracle as O
@@ -172,9 +194,14 @@ def main():
     value = cells * a.steps / (ms * 1e-3) / 1e9
     kavg_s = kms / launches * 1e-3
     achieved = bpc * cells / kavg_s / 1e9
-    traffic = None
+    traffic, traffic_src = None, None
     if a.traffic_kib:
-        traffic = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024)
+        traffic, traffic_src = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024), "--traffic-kib"
+    else:
+        traffic = profiled_traffic(w.name, a.storage)
+        if traffic is not None:
+            traffic_src = (os.path.relpath(PROFILE, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
+                           "scale; 64x4x4 launch)")
     shape = ctx.launch_shape()
     out = {
         "metric": "Gcell-updates/s (fused FDTD step, Debye media, Mur ABC)",
@@ -195,7 +222,7 @@ def main():
                    "slice_every": w.slice_every, "launch": shape, "autotuned": tuned is not None,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "fused_step", "kernel_avg_ms": round(kms / launches, 5),
                      "launches_timed": launches,
                      "bytes_per_cell": bpc},

@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--specs", default=os.path.join(ROOT, "tests", "golden", "mat_specs_riken"),
                     help="riken: tissue table")
     ap.add_argument("--requests", type=int, default=120, help="served: requests of the stream")
+    ap.add_argument("--f16-share", type=float, default=0.5,
+                    help="served: fraction of requests with fp16 storage (0: every request fp32, within "
+                         "rel-L2 1e-4 of the FP64 oracle)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served: device memory budget")
     ap.add_argument("--slots", type=int, default=64, help="served: grids on the device at once")
     ap.add_argument("--segment", type=int, default=8, help="batched/served: steps per graph replay")
@@ -440,7 +443,7 @@ def bench_served(a):
     from fdtd_amd import hip
     from fdtd_amd import workloads as W
 
-    stream = W.served_stream(a.requests)
+    stream = W.served_stream(a.requests, f16_share=a.f16_share)
     reqs = [(t, _instance(F, w, st), st) for t, w, st in stream]
     work = sum(x.params.nx * x.params.ny * x.params.nz * x.steps for _, x, _ in reqs)
     srv = F.Server(int(a.budget_gb * (1 << 30)), max_slots=a.slots, segment_steps=a.segment)

@@ -138,6 +138,11 @@ FUSED_CASES = [
     ((193, 9, 12), 8, 4, O.DEBYE_FIXED, F.JZ_STEP, 128),
     # face inside the second wave of a 128-wide tile (folded into the tile)
     ((100, 13, 11), 8, 0, O.DEBYE_FIXED, F.JZ_HOLD3, 128),
+    # the smallest grids: every edge 3 (no interior in that direction beyond one cell)
+    ((3, 3, 3), 2, 0, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((3, 9, 5), 4, 2, O.DEBYE_REF, F.JZ_STEP, 64),
+    ((70, 3, 4), 8, 0, O.DEBYE_FIXED, F.JZ_HOLD3, 64),
+    ((66, 7, 3), 2, 1, O.DEBYE_FIXED, F.JZ_STEP, 128),
 ]
 
 

@@ -177,7 +177,7 @@ def main():
     tuned = None
     if not a.no_autotune:
         # untimed: the autotuner runs each launch shape a few steps, then resets the grid to step 0
-        tuned = ctx.autotune(trial_steps=4, stream=stream.handle)
+        tuned = ctx.autotune(trial_steps=8, stream=stream.handle)
     ctx.run(a.warmup, stream.handle)
     stream.sync()
     ctx.set_timing(a.steps, every=max(1, a.time_every))

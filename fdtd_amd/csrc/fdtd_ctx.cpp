@@ -819,14 +819,16 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? 0 : fdtd_hip_fail(e, "hipEventCreate");
-    const int tys[3] = {2, 4, 8}, kcs[3] = {4, 8, 16};
+    // z-chunks 3-7 land within about 1% of each other on fp32 at 256^3 and
+    // 320x320x880 (3 best at the first, 4-6 at the second): the trials pick
+    const int tys[3] = {2, 4, 8}, kcs[5] = {3, 4, 6, 8, 16};
     // (cells per lane, threads per row): single cells 64 or 128 wide, 16-bit pairs 128 cells wide
     const int forms[3][2] = {{1, 64}, {1, 128}, {2, 64}};
     const int nform = c->esz == 2 ? 3 : 2;
     float best = 0.0f;
     int bty = c->ty, bkc = c->kchunk, bcpl = c->cpl, btxw = c->txw;
     for (int a = 0; a < 3 && !rc; a++)
-        for (int b = 0; b < 3 && !rc; b++)
+        for (int b = 0; b < 5 && !rc; b++)
             for (int f = 0; f < nform && !rc; f++) {
                 const int q = forms[f][0], w = forms[f][1];
                 set_shape(c, tys[a], kcs[b], q, w);

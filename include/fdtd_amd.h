@@ -176,7 +176,7 @@ int fdtd_probe_read(fdtd_ctx *ctx, double *host, int max_rows);
 int fdtd_slices_read(fdtd_ctx *ctx, int *steps, float *data, int max_slices);
 
 /* Autotuner: times every launch shape (tile rows {2,4,8} x z planes per
- * block {4,8,16} x tile rows of 64 or 128 threads, and two cells per lane
+ * block {3,4,6,8,16} x tile rows of 64 or 128 threads, and two cells per lane
  * under 16-bit storage) on this
  * grid for trial_steps steps each, keeps the fastest and returns the
  * context to its zero state at step 0 (probe rows and captures restart).

@@ -72,7 +72,8 @@ def parse():
     ap.add_argument("--no-sequential", action="store_true", help="batched/served: skip the sequential baselines")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="field storage (f32 reproduces the reference's arithmetic bit for bit)")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=300,
+                    help="steps of the CPU baseline sample (about 10 s of host work at 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
     ap.add_argument("--copy-bytes", type=int, default=0,

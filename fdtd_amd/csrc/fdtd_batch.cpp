@@ -660,11 +660,11 @@ extern "C" int fdtd_batch_autotune(fdtd_batch *b, int trial_steps, hipStream_t s
     hipError_t e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
     int rc = e == hipSuccess ? 0 : fdtd_hip_fail(e, "hipEventCreate");
-    const int tys[3] = {2, 4, 8}, kcs[3] = {4, 8, 16};
+    const int tys[3] = {2, 4, 8}, kcs[5] = {3, 4, 6, 8, 16};
     float best = 0.0f;
     int bty = b->ty, bkc = b->kchunk;
     for (int a = 0; a < 3 && !rc; a++)
-        for (int c = 0; c < 3 && !rc; c++) {
+        for (int c = 0; c < 5 && !rc; c++) {
             b->ty = tys[a];
             b->kchunk = kcs[c];
             b->dirty = true;

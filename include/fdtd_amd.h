@@ -279,7 +279,7 @@ int fdtd_batch_probe_read(fdtd_batch *b, int id, double *host, int max_rows);
 int fdtd_batch_get_field(fdtd_batch *b, int id, int field, int level, float *host);
 /* tile_y in {2,4,8}, kchunk >= 1, graph_steps >= 1 (0 keeps a value). */
 int fdtd_batch_set_shape(fdtd_batch *b, int tile_y, int kchunk, int graph_steps);
-/* Times every (tile_y, kchunk) candidate on the batch's own grids for
+/* Times every (tile_y in {2,4,8}, kchunk in {3,4,6,8,16}) candidate on the batch's own grids for
  * trial_steps steps each, keeps the fastest and restores the grids to
  * their zero initial state.  Every grid must be unstepped. */
 int fdtd_batch_autotune(fdtd_batch *b, int trial_steps, hipStream_t stream, int *tile_y, int *kchunk);

@@ -134,7 +134,7 @@ def test_batch_autotune_restores_state():
     ids = [b.add(x) for x in insts]
     print("\n[batch] autotune", flush=True)
     ty, kc = b.autotune(trial_steps=3)
-    assert ty in (2, 4, 8) and kc in (4, 8, 16)
+    assert ty in (2, 4, 8) and kc in (3, 4, 6, 8, 16)
     info = b.info()
     assert (info["tile_y"], info["kchunk"], info["step"]) == (ty, kc, 0)
     b.run(16)

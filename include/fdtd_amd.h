@@ -161,7 +161,11 @@ int fdtd_ctx_scales(fdtd_ctx *ctx, double *scale_e, double *scale_d, double *sca
 int fdtd_ctx_save(fdtd_ctx *ctx, const char *path);
 int fdtd_ctx_load(fdtd_ctx *ctx, const char *path);
 
-/* One full time step (H, D, sources, Debye E, Mur, probes) on `stream`. */
+/* One full time step (H, D, sources, Debye E, Mur, probes) on `stream`:
+ * the five launches of one step of the reference's loop (fdtd.cu:129-214,
+ * updateHField ... updateMurBoundary, with the ring rotation of
+ * fdtd.cu:115-530) as two launches, the fused H/D/source/E kernel and the
+ * Mur faces with the step's probe row. */
 int fdtd_step_fused(fdtd_ctx *ctx, hipStream_t stream);
 /* `steps` steps; enqueues only (no host synchronisation). */
 int fdtd_run(fdtd_ctx *ctx, int steps, hipStream_t stream);

@@ -457,13 +457,10 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
     const bool timed = c->tused < c->tcap && c->tseen++ % c->tevery == 0;
-    if (timed) HIP_TRY(hipEventRecord(c->tev[2 * c->tused], stream));
-    hipError_t e = launch_fused_step(a, c->ty, c->storage, stream);
+    hipError_t e = timed ? launch_fused_step(a, c->ty, c->storage, stream, c->tev[2 * c->tused], c->tev[2 * c->tused + 1])
+                         : launch_fused_step(a, c->ty, c->storage, stream);
     if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step");
-    if (timed) {
-        HIP_TRY(hipEventRecord(c->tev[2 * c->tused + 1], stream));
-        c->tused++;
-    }
+    if (timed) c->tused++;
 
     MurArgs m;
     m.exT = a.exT; m.eyT = a.eyT; m.ezT = a.ezT;

@@ -152,7 +152,9 @@ struct EnergyArgs {
 constexpr int kEnergyBlocks = 1024;
 hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
 
-hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s);
+// t0, t1 (optional): events that take the launch's start and end times.
+hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s, hipEvent_t t0 = nullptr,
+                             hipEvent_t t1 = nullptr);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
 // Halo planes of a z-slab: n runs of `words` 16-byte words each.

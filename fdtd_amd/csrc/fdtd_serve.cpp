@@ -271,8 +271,10 @@ void worker_main(fdtd_serve *s)
             // A device error ends the server: every admitted and queued request fails.
             std::lock_guard<std::mutex> lk(s->mu);
             for (Request *r : s->active) fail_request(s, r, rc);
-            for (auto &seg : s->inflight)
+            for (auto &seg : s->inflight) {
                 for (Request *r : seg.finished) fail_request(s, r, rc);
+                s->spare_events.push_back(seg.ev);  // destroyed with the server
+            }
             for (Request *r : s->queue) fail_request(s, r, rc);
             s->active.clear();
             s->inflight.clear();
@@ -395,10 +397,15 @@ extern "C" int fdtd_serve_wait(fdtd_serve *s, long long request, double timeout_
 {
     if (!s) return fdtd_fail(FDTD_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(request);
-    if (it == s->reqs.end()) return fdtd_fail(FDTD_EINVAL, "unknown request %lld", request);
-    Request *r = it->second.get();
-    auto done = [&] { return r->state == FDTD_REQ_DONE || r->state == FDTD_REQ_FAILED; };
+    if (s->reqs.find(request) == s->reqs.end()) return fdtd_fail(FDTD_EINVAL, "unknown request %lld", request);
+    // looked up on every wake-up: another thread may release the request
+    // once it has finished
+    auto done = [&] {
+        auto it = s->reqs.find(request);
+        if (it == s->reqs.end()) return true;
+        const int st = it->second->state;
+        return st == FDTD_REQ_DONE || st == FDTD_REQ_FAILED;
+    };
     if (timeout_s < 0) {
         s->cv_done.wait(lk, done);
         return 0;

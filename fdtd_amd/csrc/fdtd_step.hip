@@ -19,6 +19,7 @@
 // (fdtd_calculations.cu:216-428) over face cells only.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1308,49 +1309,57 @@ hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s)
 
 namespace {
 
+// One fused launch.  With events, they take the dispatch's own start and
+// end timestamps (no marker packets around it).
+hipError_t fire(const void *kernel, unsigned nblk, dim3 block, size_t lds, hipStream_t s, const StepArgs &a,
+                hipEvent_t t0, hipEvent_t t1)
+{
+    StepArgs arg = a;
+    void *args[] = {&arg};
+    return hipExtLaunchKernel(kernel, dim3(nblk), block, args, lds, s, t0, t1, 0);
+}
+
 template <typename T, bool EXACT>
-hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s)
+hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     const unsigned nblk = (unsigned)a.tiles_x * a.tiles_y * a.tiles_z;
     const size_t lds = (size_t)a.nmat * (3 * sizeof(double) + 3 * sizeof(float));
+    auto go = [&](const void *k, int w, int rows) { return fire(k, nblk, dim3(w, rows), lds, s, a, t0, t1); };
     if constexpr (sizeof(T) == 2) {
         if (a.cpl == 2) {
             switch (ty) {
-            case 2: fused_step_pair<2, T><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
-            case 4: fused_step_pair<4, T><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
-            case 8: fused_step_pair<8, T><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
+            case 2: return go((const void *)fused_step_pair<2, T>, TX, 2);
+            case 4: return go((const void *)fused_step_pair<4, T>, TX, 4);
+            case 8: return go((const void *)fused_step_pair<8, T>, TX, 8);
             default: return hipErrorInvalidValue;
             }
-            return hipGetLastError();
         }
     }
     if (a.txw == 128) {
         switch (ty) {
-        case 2: fused_step<2, T, EXACT, 128><<<nblk, dim3(128, 2), lds, s>>>(a); break;
-        case 4: fused_step<4, T, EXACT, 128><<<nblk, dim3(128, 4), lds, s>>>(a); break;
-        case 8: fused_step<8, T, EXACT, 128><<<nblk, dim3(128, 8), lds, s>>>(a); break;
+        case 2: return go((const void *)fused_step<2, T, EXACT, 128>, 128, 2);
+        case 4: return go((const void *)fused_step<4, T, EXACT, 128>, 128, 4);
+        case 8: return go((const void *)fused_step<8, T, EXACT, 128>, 128, 8);
         default: return hipErrorInvalidValue;
         }
-        return hipGetLastError();
     }
     switch (ty) {
-    case 2: fused_step<2, T, EXACT, 64><<<nblk, dim3(TX, 2), lds, s>>>(a); break;
-    case 4: fused_step<4, T, EXACT, 64><<<nblk, dim3(TX, 4), lds, s>>>(a); break;
-    case 8: fused_step<8, T, EXACT, 64><<<nblk, dim3(TX, 8), lds, s>>>(a); break;
+    case 2: return go((const void *)fused_step<2, T, EXACT, 64>, TX, 2);
+    case 4: return go((const void *)fused_step<4, T, EXACT, 64>, TX, 4);
+    case 8: return go((const void *)fused_step<8, T, EXACT, 64>, TX, 8);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s)
+hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     switch (storage) {
-    case kStoreF32: return launch_fused_t<float, true>(a, ty, s);
-    case kStoreF32Fast: return launch_fused_t<float, false>(a, ty, s);
-    case kStoreF16: return launch_fused_t<__half, false>(a, ty, s);
-    case kStoreBF16: return launch_fused_t<__hip_bfloat16, false>(a, ty, s);
+    case kStoreF32: return launch_fused_t<float, true>(a, ty, s, t0, t1);
+    case kStoreF32Fast: return launch_fused_t<float, false>(a, ty, s, t0, t1);
+    case kStoreF16: return launch_fused_t<__half, false>(a, ty, s, t0, t1);
+    case kStoreBF16: return launch_fused_t<__hip_bfloat16, false>(a, ty, s, t0, t1);
     default: return hipErrorInvalidValue;
     }
 }

@@ -53,8 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--time-every", type=int, default=16,
-                    help="HIP events around every N-th fused launch of the timed steps (each pair costs the "
-                         "stream a few microseconds)")
+                    help="time every N-th fused launch of the timed steps (HIP events bound to the dispatch)")
     ap.add_argument("--workload", default="plain_256",
                     choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken", "slabs"])
     ap.add_argument("--slabs", type=int, default=8, help="slabs: z-slabs of the 320x320x880 grid")
@@ -119,23 +118,46 @@ def profiled_traffic(workload, storage):
     return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024
 
 
+def host_cpu():
+    """The host's CPU model name, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, steps):
-    """The oracle's FP32 transcription on this host's cores (a baseline only)."""
+    """The oracle's FP32 transcription on this host's cores (a baseline only):
+    `steps` steps on up to 16 threads, then steps/25 on one thread."""
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 16)
-    O.set_threads(threads)
-    sim = O.Sim(w.shape, w.params.dt, w.params.dx, w.params.dy, w.params.dz, w.params.mu0, w.params.eps0, prec=32)
-    sim.set_materials(*w.material_arrays())
-    sim.set_sources(w.sources, w.jz, jz_mode=1)
-    sim.step(1)  # first touch of the arrays
-    t0 = time.perf_counter()
-    sim.step(steps)
-    dt = time.perf_counter() - t0
     nx, ny, nz = w.shape
+
+    def timed(nthreads, nsteps):
+        O.set_threads(nthreads)
+        sim = O.Sim(w.shape, w.params.dt, w.params.dx, w.params.dy, w.params.dz, w.params.mu0, w.params.eps0,
+                    prec=32)
+        sim.set_materials(*w.material_arrays())
+        sim.set_sources(w.sources, w.jz, jz_mode=1)
+        sim.step(1)  # first touch of the arrays
+        t0 = time.perf_counter()
+        sim.step(nsteps)
+        return time.perf_counter() - t0
+
+    dt = timed(threads, steps)
+    steps1 = max(2, steps // 25)
+    dt1 = timed(1, steps1)
     return {"value": round(nx * ny * nz * steps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{w.name} {nx}x{ny}x{nz}, {steps} steps of the oracle FP32 transcription "
-                      f"(OpenMP over z, {threads} threads), {dt:.2f} s"}
+                      f"(OpenMP over z, {threads} threads), {dt:.2f} s",
+            "single_core": {"value": round(nx * ny * nz * steps1 / dt1 / 1e9, 4), "cores": 1,
+                            "sample": f"{steps1} steps on one thread, {dt1:.2f} s"},
+            "host": {"cpu": host_cpu(), "nproc": os.cpu_count()}}
 
 
 def main():

@@ -75,6 +75,9 @@ def parse():
                     help="steps of the CPU baseline sample (about 10 s of host work at 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--shape", default=None,
+                    help="fixed launch shape TXxTYxKC (threads per tile row, tile rows, z planes per block), "
+                         "e.g. 128x2x6; no autotuning")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
@@ -99,15 +102,15 @@ def build_workload(name, steps, a=None):
 PROFILE = os.path.join(ROOT, "profiles", "fused_step_plain256_large_phantom.md")
 
 
-def profiled_traffic(workload, storage):
-    """HBM bytes per fused launch measured for this workload (rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB copy) in the committed
-    profile, or None."""
+def profiled_traffic(workload, storage, launch):
+    """HBM bytes per fused launch measured for this workload at this launch
+    shape (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB
+    copy) in the committed profile, or None."""
     if storage != "f32" or workload not in ("plain_256", "large_phantom") or not os.path.exists(PROFILE):
         return None
     import re
     text = open(PROFILE).read()
-    head = f"## {workload}:"
+    head = f"## {workload} at {launch}:"
     if head not in text:
         return None
     sec = text.split(head, 1)[1].split("\n## ", 1)[0]
@@ -191,14 +194,18 @@ def main():
     stream = hip.Stream()
     storage = {"f32": F.STORE_F32, "f16": F.STORE_F16, "bf16": F.STORE_BF16}[a.storage]
     bpc = BYTES_PER_CELL[a.storage]
-    ctx = F.Context(w.params, storage=storage)
+    fixed = {}
+    if a.shape:
+        tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
+        fixed = dict(tile_x=tx, tile_y=ty, kchunk=kc, lane_cells=1)
+    ctx = F.Context(w.params, storage=storage, **fixed)
     ctx.set_materials(w.ids, w.table)
     ctx.set_sources(w.sources, w.jz)
     ctx.set_probes(w.probes, total)
     if w.slice_every:
         ctx.set_slices(w.slice_k, w.slice_every, total // w.slice_every + 1)
     tuned = None
-    if not a.no_autotune:
+    if not (a.no_autotune or a.shape):
         # untimed: the autotuner runs each launch shape a few steps, then resets the grid to step 0
         tuned = ctx.autotune(trial_steps=8, stream=stream.handle)
     ctx.run(a.warmup, stream.handle)
@@ -220,15 +227,16 @@ def main():
     value = cells * a.steps / (ms * 1e-3) / 1e9
     kavg_s = kms / launches * 1e-3
     achieved = bpc * cells / kavg_s / 1e9
+    shape = ctx.launch_shape()
+    launch = f"{shape['tile_x']}x{shape['tile_y']}x{shape['kchunk']}"
     traffic, traffic_src = None, None
     if a.traffic_kib:
         traffic, traffic_src = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024), "--traffic-kib"
     else:
-        traffic = profiled_traffic(w.name, a.storage)
-        if traffic is not None:
-            traffic_src = (os.path.relpath(PROFILE, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
-                           "scale; 64x4x4 launch)")
-    shape = ctx.launch_shape()
+        traffic = profiled_traffic(w.name, a.storage, launch)
+        traffic_src = (os.path.relpath(PROFILE, ROOT) + f" (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
+                       f"scale; {launch} launch)") if traffic is not None else \
+            f"no counter profile of {w.name} at launch {launch} in {os.path.relpath(PROFILE, ROOT)}"
     out = {
         "metric": "Gcell-updates/s (fused FDTD step, Debye media, Mur ABC)",
         "value": round(value, 3),

@@ -148,9 +148,10 @@ int fdtd_ctx_layout(fdtd_ctx *ctx, int *pitch, long long *plane, int *elem_bytes
 int fdtd_ctx_halo_bytes(fdtd_ctx *ctx, int dir, size_t *bytes);
 int fdtd_ctx_halo_pack(fdtd_ctx *ctx, int dir, void *dst, hipStream_t stream);
 int fdtd_ctx_halo_unpack(fdtd_ctx *ctx, int dir, const void *src, hipStream_t stream);
-/* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales are 1/32 of
- * the largest per-step D/eps0 kick of the source waveform, fixed at the
- * first step. */
+/* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales, fixed at the
+ * first step, come from the source waveform: E and D/eps0 take a third of
+ * the accumulated kick max_t |sum jz| / eps0 over 4096, eta0 H 1.25 times
+ * the largest kick max |jz| / eps0 over 4096. */
 int fdtd_ctx_scales(fdtd_ctx *ctx, double *scale_e, double *scale_d, double *scale_h);
 
 /* Checkpoint and restart: write every field level, the step count and the

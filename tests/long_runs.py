@@ -1,0 +1,143 @@
+"""Long accuracy runs against the oracle: too long for the test suite, run by
+hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
+
+    python -m tests.long_runs small    # plain_small, 4000 steps: frozen 16-bit error
+    python -m tests.long_runs served   # fp16 requests of the served stream vs FP64
+    python -m tests.long_runs large    # 320x320x880, 5000 steps, fp32/fp16/bf16 vs FP64
+                                       # and the FP32 transcription (about 25 min of host time)
+
+Each prints one line per case.  The oracle is the checker here, never the
+thing measured.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import fdtd_amd as F  # noqa: E402
+from fdtd_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+STORAGES = (("fp32", F.STORE_F32), ("fp16", F.STORE_F16), ("bf16", F.STORE_BF16))
+
+
+def rel_l2(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / den) if den > 0 else float(np.linalg.norm(x - ref))
+
+
+def oracle_sim(w, prec):
+    s = O.Sim(w.shape, w.params.dt, w.params.dx, w.params.dy, w.params.dz, w.params.mu0, w.params.eps0, prec=prec)
+    s.set_materials(*w.material_arrays())
+    s.set_sources(w.sources, w.jz, jz_mode=1)
+    return s
+
+
+def gpu_ctx(w, storage, nprobe_steps):
+    c = F.Context(w.params, storage=storage)
+    c.set_materials(w.ids, w.table)
+    c.set_sources(w.sources, w.jz)
+    c.set_probes(w.probes, nprobe_steps)
+    return c
+
+
+def small():
+    """plain_small, 4000 steps: |GPU - FP64| over the largest FP64 norm of
+    Dz, Hy, Ez at each checkpoint, and the probe series."""
+    steps, checks = 4000, (150, 500, 1000, 2000, 4000)
+    w = W.plain_small(steps=steps)
+    s = oracle_sim(w, 64)
+    ref, series, done = {}, [], 0
+    for c in checks:
+        series.append(s.run(c - done, w.probes))
+        done = c
+        ref[c] = {f: s.get(f) for f in (2, 4, 8)}
+    series = np.concatenate(series)
+    peak = {f: max(np.linalg.norm(ref[c][f]) for c in checks) for f in (2, 4, 8)}
+    for name, st in STORAGES[1:]:
+        ctx = gpu_ctx(w, st, steps)
+        done, row = 0, []
+        for c in checks:
+            ctx.run(c - done)
+            done = c
+            row.append(" / ".join(f"{np.linalg.norm(ctx.get(f).astype(np.float64) - ref[c][f]) / peak[f]:.1e}"
+                                  for f in (2, 4, 8)))
+        print(f"{name} Dz / Hy / Ez at steps {checks}: " + " | ".join(row)
+              + f" | Ez probe series {rel_l2(ctx.probes()[:, 0], series[:, 0]):.1e}", flush=True)
+        ctx.close()
+
+
+def served(count=10):
+    """The first `count` fp16 requests of the served stream small enough for
+    a quick oracle run, each alone in every storage, against FP64."""
+    stream = W.served_stream(120)
+    picked = [(q, w) for q, (_, w, st) in enumerate(stream)
+              if st == "f16" and np.prod(w.shape) * w.steps <= 4e9][:count]
+    for q, w in picked:
+        ref = oracle_sim(w, 64).run(w.steps, w.probes)
+        errs = []
+        for name, st in STORAGES:
+            ctx = gpu_ctx(w, st, w.steps)
+            ctx.run(w.steps)
+            p = ctx.probes()
+            errs.append(f"{name} {rel_l2(p[:, 0], ref[:, 0]):.1e}/{rel_l2(p[:, 1], ref[:, 1]):.1e}")
+            ctx.close()
+        dist = np.linalg.norm(w.probes[0, 1:].astype(float) - w.sources[0].astype(float))
+        print(f"request {q} {w.shape} {w.steps} steps, {len(w.sources)} source cells, centre probe {dist:.0f} "
+              f"cells away; centre/near rel-L2: " + " | ".join(errs), flush=True)
+
+
+def large(steps=5000):
+    """large_phantom over `steps`: probe series and the z = nz/2 planes of Dz,
+    Hy, Ez at 1000, 2500 and `steps` against FP64 and the FP32
+    transcription."""
+    checks = tuple(c for c in (1000, 2500) if c < steps) + (steps,)
+    w = W.large_phantom(steps=steps)
+    k = w.shape[2] // 2
+
+    def run_oracle(prec):
+        s = oracle_sim(w, prec)
+        planes, series, done, t0 = {}, [], 0, time.perf_counter()
+        while done < steps:
+            n = min(100, steps - done)
+            series.append(s.run(n, w.probes))
+            done += n
+            if done in checks:
+                for f in (2, 4, 8):
+                    planes[(f, done)] = s.get(f).reshape(w.shape[::-1])[k].copy()
+            print(f"  oracle prec {prec}: step {done}, {time.perf_counter() - t0:.0f} s", flush=True)
+        return np.concatenate(series), planes
+
+    gpu = {}
+    for name, st in STORAGES:
+        ctx = gpu_ctx(w, st, steps)
+        planes, done = {}, 0
+        for c in checks:
+            ctx.run(c - done)
+            done = c
+            for f in (2, 4, 8):
+                planes[(f, c)] = ctx.get(f)[k].copy()
+        gpu[name] = (ctx.probes(), planes)
+        ctx.close()
+    p32, pl32 = run_oracle(32)
+    same = np.array_equal(gpu["fp32"][0], p32) and all(
+        np.array_equal(gpu["fp32"][1][key].view(np.uint32), pl32[key].view(np.uint32)) for key in pl32)
+    print(f"fp32 bit-equal to the FP32 transcription (probes and planes): {same}", flush=True)
+    p64, pl64 = run_oracle(64)
+    for name, (p, planes) in gpu.items():
+        probes = " ".join(f"{rel_l2(p[:, q], p64[:, q]):.1e}" for q in range(p.shape[1]))
+        pl = " | ".join(f"step {c}: " + " ".join(f"{rel_l2(planes[(f, c)], pl64[(f, c)]):.1e}" for f in (2, 4, 8))
+                        for c in checks)
+        print(f"{name} vs FP64: probes {probes}; plane Dz Hy Ez {pl}", flush=True)
+
+
+if __name__ == "__main__":
+    O.set_threads(min(os.cpu_count() or 1, 16))
+    {"small": small, "served": served, "large": large}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

@@ -1,0 +1,38 @@
+"""bench.py's contract on the CPU: more than one GPU is refused with a
+message, the committed counter profile yields the traffic figure of the
+default launch shape, and the CPU baseline carries its fields."""
+import json
+import os
+import subprocess
+import sys
+
+import bench
+from fdtd_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["n_gpus"] == 2 and "one GPU" in out["error"]
+
+
+def test_profiled_traffic_matches_the_launch_shape():
+    # the autotuned 256^3 launch and the library default both have counters
+    t = bench.profiled_traffic("plain_256", "f32", "128x2x6")
+    t0 = bench.profiled_traffic("plain_256", "f32", "64x4x4")
+    assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t
+    # a shape without counters, or 16-bit storage, gives none
+    assert bench.profiled_traffic("plain_256", "f32", "64x2x3") is None
+    assert bench.profiled_traffic("plain_256", "f16", "128x2x6") is None
+
+
+def test_cpu_baseline_fields():
+    w = W.plain_small(steps=8)
+    b = bench.cpu_baseline(w, 4)
+    assert b["value"] > 0 and b["unit"] == "Gcell-updates/s" and b["kind"] == "port"
+    assert b["cores"] >= 1 and b["single_core"]["cores"] == 1 and b["single_core"]["value"] > 0
+    assert "plain_small" in b["sample"] and b["host"]["nproc"] >= 1

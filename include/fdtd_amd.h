@@ -191,12 +191,13 @@ int fdtd_ctx_autotune(fdtd_ctx *ctx, int trial_steps, hipStream_t stream, int *t
 /* The launch shape chosen for the context's fused kernel. */
 int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, int *blocks);
 
-/* Records a HIP event pair around every `every`-th fused-kernel launch
- * (the first, then each `every` launches; every >= 1), up to `capacity`
- * pairs, on the stream the kernel runs on (capacity 0 disables).  An event
- * pair costs the stream a few microseconds, so sampling keeps the timed
- * run's wall clock close to an untimed one.  fdtd_ctx_kernel_time waits
- * for the recorded pairs and returns their summed duration. */
+/* Times every `every`-th fused-kernel launch (the first, then each `every`
+ * launches; every >= 1), up to `capacity` of them (capacity 0 disables):
+ * a HIP event pair bound to the launch's dispatch takes the kernel's own
+ * start and end.  Timing every launch costs the step sequence about 1.5%,
+ * so sampling keeps the timed run's wall clock close to an untimed one.
+ * fdtd_ctx_kernel_time waits for the recorded pairs and returns their
+ * summed duration. */
 int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity, int every);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
 

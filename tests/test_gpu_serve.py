@@ -58,3 +58,49 @@ def test_serve_admits_against_budget():
     with pytest.raises(F.FdtdError, match="-5"):
         srv.submit(make_case(9, (96, 96, 96), 4, 5))   # larger than the whole budget
     srv.close()
+
+
+def test_serve_concurrent_clients_and_release():
+    """Four client threads submit, wait for and release their own requests
+    while the worker runs them together; each result equals a single
+    context's.  A running request cannot be released, and a released one is
+    unknown afterwards."""
+    import threading
+    cases = [make_case(700 + q, s, 5, n, storage=st) for q, (s, n, st) in enumerate([
+        ((24, 20, 18), 21, F.STORE_F32), ((30, 16, 22), 17, F.STORE_F16), ((18, 26, 20), 29, F.STORE_F32),
+        ((20, 20, 30), 13, F.STORE_BF16), ((34, 18, 16), 25, F.STORE_F32), ((22, 22, 22), 19, F.STORE_F32_FAST),
+        ((16, 30, 24), 23, F.STORE_F32), ((26, 24, 14), 15, F.STORE_F16)])]
+    want = [single(c)[1] for c in cases]
+    srv = F.Server(1 << 30, max_slots=6, segment_steps=4)
+    print("\n[serve] 4 client threads x 2 requests", flush=True)
+    errors = []
+
+    def client(t):
+        try:
+            for q in (2 * t, 2 * t + 1):
+                r = srv.submit(cases[q])
+                assert srv.wait(r, 120.0)
+                got, inf = srv.result(r)
+                assert inf["state"] == F.REQ_DONE, inf
+                assert series_equal(got, want[q]), f"case {q}"
+                srv.release(r)
+        except Exception as e:  # reported by the main thread
+            errors.append(f"client {t}: {e!r}")
+
+    threads = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(300.0)
+    assert not errors, errors
+    st = srv.stats()
+    assert st["submitted"] == 8 and st["completed"] == 8 and st["failed"] == 0
+    long_run = make_case(800, (48, 40, 36), 5, 2000)
+    r = srv.submit(long_run)
+    with pytest.raises(F.FdtdError, match="-3"):
+        srv.release(r)                      # still queued or running
+    assert srv.wait(r, 120.0)
+    srv.release(r)
+    with pytest.raises(F.FdtdError, match="-1"):
+        srv.wait(r, 0.0)                    # released: unknown
+    srv.close()

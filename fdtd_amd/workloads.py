@@ -31,6 +31,10 @@ Workloads (sizes as stated):
                 (data/input_params: 3 GHz, pulse width 0.5, no carrier
                 modulation, 100 cells per wavelength, so 1 mm cells) at
                 Courant 0.99, a soft Jz source at the grid centre.
+  box_phantom   any grid: vacuum with four seeded axis-aligned Debye boxes
+                kept PHANTOM_MARGIN cells inside the faces, a centred Ez point
+                source at 900 MHz and a centre probe; cheap to build at
+                billions of cells.
   served_stream Poisson arrivals (mean 40 per second) of solves with edges
                 in [32,192] per axis, 200-4000 steps, fp32 or fp16 storage
                 (even odds), 4-12 material ellipsoid phantoms and a point or
@@ -315,6 +319,26 @@ def batched_set(count=64, seed=12349):
         steps = rng.randint(500, 3000)
         out.append(_grid(f"batched_{q}", rng, shape, steps, freq, nmat, False, seed))
     return out
+
+
+def box_phantom(shape, steps=100, seed=12351):
+    """Vacuum plus four seeded Debye boxes (see module doc); painted by
+    slicing, so a grid of 10^9 cells builds in seconds."""
+    nx, ny, nz = shape
+    params, jz = derive(shape, steps, 9.0e8, 2.0e-3, 0.99)
+    rng = SplitMix64(seed)
+    table = np.concatenate([np.asarray([(0.0, 1.0, 1.0, 0.0)], np.float32), debye_table(rng, 4)])
+    ids = np.zeros((nz, ny, nx), np.uint8)
+    M = PHANTOM_MARGIN
+    for m in range(1, 5):
+        lo = [M + int(rng.uniform(0.0, 0.4) * (n - 2 * M)) for n in (nx, ny, nz)]
+        hi = [min(n - M, l + max(1, int(rng.uniform(0.1, 0.5) * (n - 2 * M)))) for l, n in zip(lo, (nx, ny, nz))]
+        ids[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = m
+    c = (nx // 2, ny // 2, nz // 2)
+    src = np.asarray([c], np.int32)
+    probes = np.asarray([[8, c[0], c[1], c[2]], [8, c[0] + 8, c[1], c[2]]], np.int32)
+    return Workload("box_phantom", shape, params, ids, table, src, jz[:max(steps, 1)].copy(), probes, steps,
+                    seed=seed, freq=9.0e8)
 
 
 def served_stream(count=120, seed=12350, rate=40.0, f16_share=0.5):

@@ -38,3 +38,18 @@ def test_served_stream_arrivals_and_mix():
         assert storage in ("f32", "f16")
         if w.ids.any():
             assert face_margin(w.ids) >= W.PHANTOM_MARGIN
+
+
+def test_box_phantom_margins_and_determinism():
+    a = W.box_phantom((64, 48, 40), steps=10)
+    b = W.box_phantom((64, 48, 40), steps=10)
+    assert np.array_equal(a.ids, b.ids)
+    M = W.PHANTOM_MARGIN
+    inner = a.ids.copy()
+    inner[M:-M, M:-M, M:-M] = 0
+    assert not inner.any()                     # nothing within the margin of a face
+    assert set(np.unique(a.ids)) <= set(range(5)) and a.ids.max() > 0
+    nx, ny, nz = a.shape
+    for x, y, z in a.sources:
+        assert 1 <= x < nx - 1 and 1 <= y < ny - 1 and 0 <= z < nz - 1
+    assert a.table.shape == (5, 4) and len(a.jz) == 10

@@ -36,3 +36,11 @@ def test_cpu_baseline_fields():
     assert b["value"] > 0 and b["unit"] == "Gcell-updates/s" and b["kind"] == "port"
     assert b["cores"] >= 1 and b["single_core"]["cores"] == 1 and b["single_core"]["value"] > 0
     assert "plain_small" in b["sample"] and b["host"]["nproc"] >= 1
+
+
+def test_long_runs_script_imports():
+    """tests/long_runs.py (run by hand on a GPU host) stays importable and
+    exposes its three runs."""
+    import tests.long_runs as L
+    assert callable(L.small) and callable(L.served) and callable(L.large)
+    assert L.rel_l2([1.0, 2.0], [1.0, 2.0]) == 0.0

@@ -916,7 +916,7 @@ __device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, Step
 // instances outside their step window are skipped.
 // The paired 16-bit form fits four waves per SIMD in 128 VGPRs without
 // spilling (left alone it takes 142 and three).  The scaled-unit one-cell
-// forms run five waves per SIMD in 96 VGPRs with a small spill (32-60 bytes
+// forms run five waves per SIMD in 96 VGPRs with a small spill (0-12 bytes
 // per lane), measured 5% faster on 64 uneven fp16 grids than four waves;
 // the exact fp32 form gains nothing from it and keeps four without spilling.
 template <int TY, typename T, bool EXACT, bool PAIR>

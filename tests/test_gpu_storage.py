@@ -150,3 +150,21 @@ def test_far_field_of_a_point_source_fp16():
     print(f"\nserved request 13, fp16: probe rel-L2 vs FP64 (centre, near the source) {err[0]:.2e} {err[1]:.2e}")
     assert np.isfinite(got).all()
     assert err[0] < 1e-1 and err[1] < 3e-2
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16, F.STORE_F32_FAST])
+def test_auto_scales_follow_the_source_bounds(storage):
+    """The automatic scales map a third of the accumulated kick (E, D/eps0)
+    and 1.25 times the largest kick (eta0 H) to 4096 storage units."""
+    w = W.plain_small()
+    ctx = F.Context(w.params, storage=storage)
+    ctx.set_materials(w.ids, w.table)
+    ctx.set_sources(w.sources, w.jz)
+    sE, sD, sH = ctx.scales()
+    eps0 = float(np.float32(w.params.eps0))
+    acc = np.abs(np.cumsum(w.jz.astype(np.float64))).max()
+    peak = np.abs(w.jz.astype(np.float64)).max()
+    assert sE == pytest.approx(acc / eps0 / 3.0 / 4096.0, rel=1e-12)
+    assert sD == sE
+    assert sH == pytest.approx(1.25 * peak / eps0 / 4096.0, rel=1e-12)
+    ctx.close()

@@ -1,0 +1,58 @@
+"""Device memory comes back: contexts (with probes, slices, energy and
+timing), batches (with instances added, stepped and removed) and servers
+created and destroyed repeatedly leave the free device memory where it was."""
+import numpy as np
+import pytest
+
+import fdtd_amd as F
+from fdtd_amd import hip
+from tests.helpers import EPS0, MU0, random_materials
+from tests.test_gpu_batch import make_case
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (48, 40, 36)
+
+
+def _cycle_context(seed):
+    rng = np.random.default_rng(seed)
+    ids, table = random_materials(rng, SHAPE)
+    c = F.Context(F.make_params(SHAPE, 3.8e-12, 2e-3, mu0=MU0, eps0=EPS0), storage=[F.STORE_F32, F.STORE_F16][seed % 2])
+    c.set_materials(ids, table)
+    c.set_sources(np.asarray([[20, 20, 15]], np.int32), np.sin(np.arange(30) * 0.3).astype(np.float32))
+    c.set_probes(np.asarray([[8, 24, 20, 18]], np.int32), 20)
+    c.set_slices(18, 5, 4)
+    c.set_energy(5, 4)
+    c.set_timing(4, 5)
+    c.run(20)
+    c.probes()
+    c.close()
+
+
+def test_create_destroy_returns_device_memory():
+    print("\ncontext/batch/server lifetimes", flush=True)
+    hip.sync()
+    _cycle_context(0)   # first use loads the code objects
+    hip.sync()
+    free0 = hip.mem_info()[0]
+    for seed in range(12):
+        _cycle_context(seed)
+    b = F.Batch(256 << 20)
+    ids = [b.add(make_case(900 + q, (24 + 4 * q, 20, 18), 4, 12)) for q in range(4)]
+    b.run(6)
+    b.remove(ids[1])
+    b.add(make_case(950, (30, 22, 20), 4, 9))
+    b.run(20)
+    b.close()
+    s = F.Server(256 << 20, max_slots=4, segment_steps=4)
+    rs = [s.submit(make_case(960 + q, (20, 20, 20), 4, 10)) for q in range(3)]
+    assert s.drain(60.0)
+    for r in rs:
+        s.release(r)
+    s.close()
+    hip.sync()
+    free1 = hip.mem_info()[0]
+    print(f"free device memory before {free0}, after {free1} bytes", flush=True)
+    # allocator granularity aside, nothing stays behind (one leaked context of
+    # this size would hold 15 MB, a batch or server arena 256 MB)
+    assert abs(free0 - free1) < (8 << 20), (free0, free1)

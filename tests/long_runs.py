@@ -5,6 +5,9 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
     python -m tests.long_runs served   # fp16 requests of the served stream vs FP64
     python -m tests.long_runs large    # 320x320x880, 5000 steps, fp32/fp16/bf16 vs FP64
                                        # and the FP32 transcription (about 25 min of host time)
+    python -m tests.long_runs bounds   # peak fields of soft Jz sources against the accumulated
+                                       # and the largest kick (the basis of the 16-bit scales;
+                                       # oracle only, no GPU)
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -138,6 +141,37 @@ def large(steps=5000):
         print(f"{name} vs FP64: probes {probes}; plane Dz Hy Ez {pl}", flush=True)
 
 
+def bounds(n=40):
+    """FP64 oracle in an n^3 box: the largest |E|, |D|/eps0 and eta0 |H| a
+    soft Jz source drives, over K = max_t |sum jz| / eps0 (the accumulated
+    kick) and over the largest single kick max |jz| / eps0, for point, 10-cell
+    line and full-plane sources at 0.3-6 GHz, in vacuum and at eps_r = 50."""
+    c = n // 2
+    shapes = {"point": [[c, c, c]], "line10": [[c, c, z] for z in range(c - 5, c + 5)],
+              "sheet": [[10, y, z] for y in range(1, n - 1) for z in range(0, n - 1)]}
+    cases = [(f, k, 1.0) for f in (0.3e9, 0.9e9, 3e9, 6e9) for k in ("point", "line10")]
+    cases += [(0.9e9, "sheet", 1.0), (3e9, "sheet", 1.0), (0.9e9, "point", 50.0)]
+    for freq, kind, eps in cases:
+        steps = 1200 if freq < 0.5e9 else 600
+        params, jz = W.derive((n, n, n), steps, freq, 2.0e-3, 0.99)
+        eta0 = np.sqrt(params.mu0 / params.eps0)
+        kick = np.abs(jz).max() / params.eps0
+        K = np.abs(np.cumsum(jz.astype(np.float64))).max() / params.eps0
+        s = O.Sim((n, n, n), params.dt, params.dx, params.dy, params.dz, params.mu0, params.eps0, prec=64)
+        z = np.zeros((n, n, n), np.float32)
+        s.set_materials(z, z + eps, z + eps, z)
+        s.set_sources(np.asarray(shapes[kind], np.int32), jz, jz_mode=1)
+        last = int(np.flatnonzero(jz).max()) if np.any(jz) else 0
+        mE = mD = mH = 0.0
+        for _ in range(0, min(steps, last + 100), 10):
+            s.step(10)
+            mE = max(mE, max(np.abs(s.get(f)).max() for f in (6, 7, 8)))
+            mD = max(mD, max(np.abs(s.get(f)).max() for f in (0, 1, 2)) / params.eps0)
+            mH = max(mH, max(np.abs(s.get(f)).max() for f in (3, 4, 5)) * eta0)
+        print(f"{freq / 1e9:4.1f} GHz {kind:6s} eps_r {eps:4.1f}: max E/K {mE / K:.3f}, D/eps0/K {mD / K:.3f}, "
+              f"eta0 H/kick {mH / kick:.3f}", flush=True)
+
+
 if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
-    {"small": small, "served": served, "large": large}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+    {"small": small, "served": served, "large": large, "bounds": bounds}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--time-every", type=int, default=16,
                     help="time every N-th fused launch of the timed steps (HIP events bound to the dispatch)")
     ap.add_argument("--workload", default="plain_256",
-                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken", "slabs"])
+                    choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken", "slabs",
+                             "stream"])
     ap.add_argument("--slabs", type=int, default=8, help="slabs: z-slabs of the 320x320x880 grid")
     ap.add_argument("--grid", default=None,
                     help="slabs: NXxNYxNZ of a box phantom in place of the 320x320x880 grid (any size that "
@@ -188,6 +189,8 @@ def main():
         return bench_served(a)
     if a.workload == "slabs":
         return bench_slabs(a)
+    if a.workload == "stream":
+        return bench_stream(a)
 
     total = a.warmup + a.steps
     w = build_workload(a.workload, total, a)
@@ -268,6 +271,30 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps)
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_stream(a):
+    """The memory ceiling of the fused step's access mix: 15 arrays read and
+    9 written per cell (256^3 floats each), no arithmetic, every access
+    nontemporal (fdtd_stream_probe), at the resident block count and at two
+    larger grids."""
+    import fdtd_amd as F
+    cells = 256 ** 3
+    runs = {}
+    for blocks in (0, 4096, 8192):
+        gbs, ms = F.stream_probe(cells, blocks, reps=max(1, a.steps // 100))
+        runs["resident" if blocks == 0 else str(blocks)] = {"gb_per_s": round(gbs, 1), "ms_per_pass": round(ms, 5)}
+    best = max(v["gb_per_s"] for v in runs.values())
+    out = {
+        "metric": "GB/s streaming the fused step's 15-read / 9-write access mix without arithmetic",
+        "value": best, "unit": "GB/s", "n_gpus": 1, "steps": max(1, a.steps // 100), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (zeros)",
+        "config": {"workload": "stream_mix", "cells": cells, "arrays": {"read": 15, "written": 9},
+                   "parallelism": "single GPU"},
+        "by_grid": runs, "frac_of_peak": round(best / HBM_PEAK_GBS, 4),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -52,7 +52,7 @@ EXPORTS = {
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
-                   "fdtd_dropin_solve", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
+                   "fdtd_dropin_solve", "fdtd_stream_probe", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
@@ -192,6 +192,7 @@ def lib():
         L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
         L.fdtd_dropin_solve.argtypes = [PP, PE, PI, vp, vp]
+        L.fdtd_stream_probe.argtypes = [C.c_longlong, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fdtd_load_tissue_table.argtypes = [C.c_char_p, vp]
         L.fdtd_load_phantom_ids.argtypes = [C.c_char_p, i, i, i, i, i, vp]
         L.fdtd_serve_create.argtypes = [C.POINTER(FdtdServeConfig), C.POINTER(vp)]
@@ -462,6 +463,15 @@ class Instance:
         check(lib().fdtd_batch_instance_bytes(C.byref(self.params), C.byref(self.ext), C.byref(self.c),
                                               C.byref(b)), "instance_bytes")
         return b.value
+
+
+def stream_probe(cells, blocks=0, reps=20):
+    """(GB/s, ms per pass) of the fused step's 15-read / 9-write access mix
+    streamed without arithmetic over `cells` floats per array
+    (fdtd_stream_probe); blocks 0 = as many as fit resident."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().fdtd_stream_probe(int(cells), int(blocks), int(reps), C.byref(g), C.byref(ms)), "fdtd_stream_probe")
+    return g.value, ms.value
 
 
 def load_tissue_table(path):

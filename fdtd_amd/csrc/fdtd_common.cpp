@@ -321,3 +321,39 @@ void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, cons
             out1[q] = one(rpxEnd[r + 1], rpxEnd[r]);
         }
 }
+
+extern "C" int fdtd_stream_probe(long long cells, int blocks, int reps, double *gbs, double *ms_per_pass)
+{
+    if (cells < 1 || reps < 1 || !gbs) return fdtd_fail(FDTD_EINVAL, "stream probe: cells %lld, reps %d", cells, reps);
+    if (blocks <= 0) blocks = stream_mix_resident_blocks();
+    if (blocks <= 0) return fdtd_fail(FDTD_EINVAL, "stream probe: no resident blocks");
+    const size_t stride = (size_t)cells + 65536;  // the contexts' stagger between arrays
+    float *blob = nullptr;
+    HIP_TRY(hipMalloc(&blob, 24 * stride * sizeof(float)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    StreamMix p;
+    for (int a = 0; a < 24; a++) {
+        float *b = blob + a * stride;
+        if (a < 15) p.r[a] = b;
+        else p.w[a - 15] = b;
+    }
+    p.n = cells;
+    hipError_t e = hipMemset(blob, 0, 24 * stride * sizeof(float));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_stream_mix(p, blocks, nullptr);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_stream_mix(p, blocks, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(blob);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "stream probe");
+    const double per = (double)ms / reps;
+    *gbs = 24.0 * sizeof(float) * (double)cells / (per * 1e-3) / 1e9;
+    if (ms_per_pass) *ms_per_pass = per;
+    return 0;
+}

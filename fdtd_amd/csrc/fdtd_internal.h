@@ -152,6 +152,16 @@ struct EnergyArgs {
 constexpr int kEnergyBlocks = 1024;
 hipError_t launch_energy(const EnergyArgs &a, int storage, hipStream_t s);
 
+// Streaming probe (fdtd_stream_probe): 15 read and 9 written arrays of n floats.
+struct StreamMix {
+    const float *r[15];
+    float *w[9];
+    long long n;
+};
+hipError_t launch_stream_mix(const StreamMix &p, int blocks, hipStream_t s);
+// Blocks of the streaming probe resident on the whole device at once.
+int stream_mix_resident_blocks();
+
 // t0, t1 (optional): events that take the launch's start and end times.
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s, hipEvent_t t0 = nullptr,
                              hipEvent_t t1 = nullptr);

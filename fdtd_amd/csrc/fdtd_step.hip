@@ -1447,3 +1447,36 @@ hipError_t launch_batch_advance(BatchShared *sh, int steps, hipStream_t s)
     batch_advance<<<1, 64, 0, s>>>(sh, steps);
     return hipGetLastError();
 }
+
+namespace {
+
+// The fused step's memory stream without its arithmetic: 15 arrays read and
+// 9 written, element for element, every access nontemporal; a grid-stride
+// loop over n elements of each array.
+__global__ __launch_bounds__(256) void stream_mix(const StreamMix p)
+{
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < p.n; q += (long long)gridDim.x * 256) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 15; a++) acc += __builtin_nontemporal_load(&p.r[a][q]);
+#pragma unroll
+        for (int b = 0; b < 9; b++) __builtin_nontemporal_store(acc, &p.w[b][q]);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_stream_mix(const StreamMix &p, int blocks, hipStream_t s)
+{
+    stream_mix<<<blocks, 256, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+int stream_mix_resident_blocks()
+{
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, stream_mix, 256, 0) != hipSuccess) return 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return nb * cus;
+}

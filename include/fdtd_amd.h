@@ -201,6 +201,15 @@ int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, 
 int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity, int every);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
 
+/* Memory-ceiling calibration (no counterpart in the reference): streams the
+ * fused step's access mix without its arithmetic, 15 arrays read and 9
+ * written element for element (cells floats each, 64 Ki elements apart as
+ * in a context, every access nontemporal), `reps` timed passes on `blocks`
+ * blocks of 256 threads (0 = as many as fit resident).  Allocates and frees
+ * 96 bytes per cell.  gbs receives 96 * cells bytes over the mean pass
+ * time, ms_per_pass (optional) that time. */
+int fdtd_stream_probe(long long cells, int blocks, int reps, double *gbs, double *ms_per_pass);
+
 /* ---- voxel phantoms (fdtd.cu:875-939) ------------------------------------ */
 
 #define FDTD_PGM_REFERENCE 0  /* nx*ny codes in file order whatever the image width (fdtd.cu:920-933) */

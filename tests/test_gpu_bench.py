@@ -40,3 +40,12 @@ def test_bench_slabs_box_phantom():
     assert d["config"]["workload"] == "box_phantom_slabs" and d["config"]["slab_planes"] == [20, 20, 20]
     assert d["value"] > 0 and d["unsplit_value"] > 0 and d["exchange_us_per_step"] > 0
     assert d["message_bytes_per_interface"]["up"] == 5 * 96 * 64 * 4   # five pitched planes (96 is a line multiple)
+
+
+def test_stream_probe():
+    import fdtd_amd as F
+    print("stream probe, 1 M cells", flush=True)
+    gbs, ms = F.stream_probe(1 << 20, 0, 4)
+    assert 100.0 < gbs < 9000.0 and ms > 0.0
+    with pytest.raises(F.FdtdError, match="-1"):
+        F.stream_probe(0)

@@ -26,6 +26,9 @@ Two more workloads measure the systems patterns:
                       p50/p99 latency beside the sequential drop-in path (the
                       five reference entry points, one solve at a time, FIFO)
                       on the same stream.
+  --workload stream   the memory ceiling of the fused step's access mix: 15
+                      arrays read and 9 written per cell with no arithmetic
+                      (fdtd_stream_probe), in GB/s.
   --workload slabs    320x320x880 cut into --slabs z-slabs (each with ghost
                       planes), all stepped on this one GPU with the halo
                       exchange of every interface before every step: the

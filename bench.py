@@ -271,9 +271,15 @@ def main():
                      "bytes_per_cell": bpc},
         "probes_finite_nonzero": ok,
     }
+    ctx.close()
+    if a.storage == "f32":
+        # the same box's streaming ceiling for this access mix (fdtd_stream_probe), a yardstick beside the peak
+        ceil_gbs, _ = F.stream_probe(cells, 0, reps=10)
+        out["roofline"]["stream_ceiling"] = {"gb_per_s": round(ceil_gbs, 1),
+                                             "frac": round(achieved / ceil_gbs, 4),
+                                             "what": "15 arrays read + 9 written per cell, no arithmetic"}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps)
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -32,6 +32,8 @@ def test_bench_line_contract():
     assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["launches_timed"] == 8
     assert 0 < r["frac"] < 1 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
     assert r["traffic"] is None      # no counter profile for this workload
+    c = r["stream_ceiling"]
+    assert c["gb_per_s"] > 0 and abs(c["frac"] - r["achieved"] / c["gb_per_s"]) < 1e-3
     assert d["config"]["workload"] == "plain_small" and d["config"]["grid"] == [64, 64, 64]
 
 

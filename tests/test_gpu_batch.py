@@ -159,3 +159,21 @@ def test_dropin_solve_equals_fused():
         got = F.dropin_solve(inst)
         want = single(inst)[1]
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), f"jz mode {jm}"
+
+
+def test_batch_reference_debye_mode():
+    """Grids in the reference's Debye mode (d = 2b - sigma dt^2, unstable but
+    short runs stay finite) and the stable default in one batch: each equals
+    its single context bit for bit, fp32 and fp16."""
+    insts = [make_case(400, (22, 18, 26), 6, 24, debye=F.DEBYE_REF),
+             make_case(401, (30, 14, 20), 5, 19, storage=F.STORE_F16, debye=F.DEBYE_REF),
+             make_case(402, (18, 26, 16), 4, 21)]
+    refs = [single(x) for x in insts]
+    b = F.Batch(1 << 28)
+    print("\n[batch] REF and FIXED Debye modes together", flush=True)
+    ids = [b.add(x) for x in insts]
+    b.run(30)
+    for q, (i, x, r) in enumerate(zip(ids, insts, refs)):
+        assert_same(b, i, x, r, f"debye case {q}")
+    assert np.isfinite(refs[0][1]).all()
+    b.close()

@@ -88,3 +88,24 @@ def test_snapshot_restart_is_bit_exact(tmp_path, storage):
     other = F.Context(F.make_params((36, 21, 18), DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
     with pytest.raises(F.FdtdError, match="-2"):
         other.load(tmp_path / "state.bin")
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16])
+def test_energy_of_16bit_fields(storage):
+    """Under 16-bit storage the device folds the scales into its FP64 sum:
+    it equals the sum of E.D/2 + mu0 |H|^2/2 over the stored fields read back
+    in SI units (each read-back value rounded to fp32, hence 1e-6)."""
+    shape = (30, 22, 26)
+    rng = np.random.default_rng(12)
+    ids, table = random_materials(rng, shape, 5)
+    ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
+    ctx.set_materials(ids, table)
+    ctx.set_sources(random_sources(rng, shape, 3), _jz())
+    ctx.set_energy(10, 10)
+    ctx.run(30)
+    _, energy = ctx.energy()
+    f = {q: ctx.get(q).astype(np.float64) for q in range(9)}
+    want = 0.5 * sum((f[6 + c] * f[c]).sum() for c in range(3)) + 0.5 * float(np.float32(MU0)) * sum(
+        (f[3 + c] ** 2).sum() for c in range(3))
+    assert want > 0
+    np.testing.assert_allclose(energy[-1], want, rtol=1e-6, atol=0)

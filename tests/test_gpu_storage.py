@@ -168,3 +168,29 @@ def test_auto_scales_follow_the_source_bounds(storage):
     assert sD == sE
     assert sH == pytest.approx(1.25 * peak / eps0 / 4096.0, rel=1e-12)
     ctx.close()
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16])
+def test_16bit_slices_and_probes_read_the_stored_fields(storage):
+    """Under 16-bit storage the z-slices and probe rows leave the device in
+    SI units: the last slice equals the read-back fields bit for bit (same
+    decoding), and each probe row equals them to fp32 rounding (probes are
+    decoded in double)."""
+    w = W.plain_small(steps=30)
+    probes = np.asarray([[8, 32, 32, 32], [4, 20, 30, 28], [0, 40, 33, 31], [6, 30, 30, 30]], np.int32)
+    ctx = F.Context(w.params, storage=storage)
+    ctx.set_materials(w.ids, w.table)
+    ctx.set_sources(w.sources, w.jz)
+    ctx.set_probes(probes, 30)
+    ctx.set_slices(32, 10, 4)
+    ctx.run(30)
+    steps, data = ctx.slices()
+    assert list(steps) == [10, 20, 30]
+    fields = [ctx.get(f) for f in range(9)]
+    for f in range(9):
+        assert np.array_equal(data[-1][f].view(np.uint32), fields[f][32].view(np.uint32)), O.FIELDS[f]
+    last = ctx.probes()[-1]
+    for p, (f, i, j, k) in enumerate(probes):
+        want = float(fields[f][k, j, i])
+        assert abs(last[p] - want) <= 1e-6 * abs(want) + 1e-30, (p, last[p], want)
+    assert np.abs(last).max() > 0

@@ -56,7 +56,7 @@ EXPORTS = {
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
-ENOMEM = -5
+EINVAL, ERANGE, ESTATE, EIO, ENOMEM = -1, -2, -3, -4, -5
 PGM_REFERENCE, PGM_IMAGE = 0, 1
 HALO_UP, HALO_DOWN = 0, 1
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3

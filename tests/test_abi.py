@@ -83,3 +83,14 @@ def test_extension_struct_layout(tmp_path):
         assert int(got[name]) == C.sizeof(S), name
         for f in fields:
             assert int(got[f"{name}.{f}"]) == getattr(S, f).offset, f"{name}.{f}"
+
+
+def test_constants_match_header():
+    """Every FDTD_* value the package mirrors equals the header's #define."""
+    text = open(os.path.join(INCLUDE, "fdtd_amd.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define FDTD_(\w+)\s+\(?(-?\d+)\)?", text, re.M)}
+    mirrored = [n for n in defs if hasattr(F, n)]
+    assert len(mirrored) >= 19, mirrored
+    for n in mirrored:
+        assert getattr(F, n) == defs[n], n
+    assert [defs[f"{f.upper()}"] for f in F.FIELDS] == list(range(9))

@@ -104,3 +104,22 @@ def test_serve_concurrent_clients_and_release():
     with pytest.raises(F.FdtdError, match="-1"):
         srv.wait(r, 0.0)                    # released: unknown
     srv.close()
+
+
+def test_serve_failed_admission_leaves_the_rest():
+    good = [make_case(900 + q, (32, 28, 24), 5, 20 + q) for q in range(3)]
+    bad = make_case(905, (32, 28, 24), 5, 20)
+    bad.ids[7] = 5                          # material id == nmat: rejected when the grid is added
+    srv = F.Server(1 << 30, max_slots=8, segment_steps=4)
+    print("\n[serve] one request with a material id past its table", flush=True)
+    rs = [srv.submit(good[0]), srv.submit(bad), srv.submit(good[1]), srv.submit(good[2])]
+    assert srv.drain(120.0)
+    got, inf = srv.result(rs[1])
+    assert inf["state"] == F.REQ_FAILED and inf["error"] == F.ERANGE and inf["rows"] == 0 and got.shape[0] == 0
+    for rq, inst in zip(rs[:1] + rs[2:], good):
+        series, inf = srv.result(rq)
+        assert inf["state"] == F.REQ_DONE
+        assert series_equal(series, single(inst)[1])
+    st = srv.stats()
+    assert st["completed"] == 3 and st["failed"] == 1 and st["running"] == 0
+    srv.close()

@@ -481,6 +481,7 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
         he = hipMemcpyAsync(base + L.probes, in->probes, sizeof(int) * 4 * in->nprobe, hipMemcpyHostToDevice, b->up);
     if (he == hipSuccess) he = hipStreamSynchronize(b->up);
     if (he != hipSuccess) {
+        (void)hipStreamSynchronize(b->up);   // copies queued before the failure still read the host vectors
         b->arena.release(off, L.total);
         return fdtd_hip_fail(he, "fdtd_batch_add upload");
     }

@@ -271,7 +271,8 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
     if (c->slice_host) (void)hipHostFree(c->slice_host);
     if (c->slice_ev) (void)hipEventDestroy(c->slice_ev);
     if (c->side) (void)hipStreamDestroy(c->side);
-    for (auto ev : c->tev) (void)hipEventDestroy(ev);
+    for (auto ev : c->tev)
+        if (ev) (void)hipEventDestroy(ev);
     delete c;
     return 0;
 }
@@ -282,11 +283,17 @@ extern "C" int fdtd_ctx_set_materials(fdtd_ctx *c, const unsigned char *ids, int
     if (nmat < 1 || nmat > kMaxMaterials) return fdtd_fail(FDTD_ERANGE, "nmat %d not in [1,%d]", nmat, kMaxMaterials);
     for (size_t t = 0; t < c->N; t++)
         if (ids[t] >= nmat) return fdtd_fail(FDTD_ERANGE, "material id %d at cell %zu >= nmat %d", ids[t], t, nmat);
+    // the device ids first: the table changes only once they match it
+    std::vector<unsigned char> prev(ids, ids + c->N);
+    c->ids_host.swap(prev);
+    if (int rc = upload_ids(c)) {
+        c->ids_host.swap(prev);
+        return rc;
+    }
     c->table.assign(table, table + 4 * nmat);
     c->nmat = nmat;
     c->lut_dirty = true;
-    c->ids_host.assign(ids, ids + c->N);
-    return upload_ids(c);
+    return 0;
 }
 
 extern "C" int fdtd_ctx_set_material_arrays(fdtd_ctx *c, const float *sigma, const float *epsS, const float *epsI,
@@ -349,14 +356,16 @@ extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, i
     (void)hipFree(c->probe_out);
     c->probes = nullptr;
     c->probe_out = nullptr;
-    c->nprobe = nprobe;
-    c->probe_cap = capacity;
+    c->nprobe = 0;                     // stays 0 (no probe launches) unless every step below succeeds
+    c->probe_cap = 0;
     c->probe_rows = 0;
-    if (nprobe == 0 || capacity == 0) { c->nprobe = 0; return 0; }
+    if (nprobe == 0 || capacity == 0) return 0;
     HIP_TRY(hipMalloc(&c->probes, sizeof(int) * 4 * nprobe));
     HIP_TRY(hipMemcpy(c->probes, probes, sizeof(int) * 4 * nprobe, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->probe_out, sizeof(double) * (size_t)nprobe * capacity));
     HIP_TRY(hipMemset(c->probe_out, 0, sizeof(double) * (size_t)nprobe * capacity));
+    c->nprobe = nprobe;
+    c->probe_cap = capacity;
     return 0;
 }
 
@@ -370,16 +379,18 @@ extern "C" int fdtd_ctx_set_slices(fdtd_ctx *c, int k, int interval, int capacit
     c->slice_stage = nullptr;
     c->slice_host = nullptr;
     c->slice_k = k;
-    c->slice_interval = interval;
-    c->slice_cap = capacity;
+    c->slice_interval = 0;             // stays 0 (no capture) unless every step below succeeds
+    c->slice_cap = 0;
     c->slice_count = 0;
     c->slice_steps.clear();
-    if (interval == 0 || capacity == 0) { c->slice_interval = 0; return 0; }
+    if (interval == 0 || capacity == 0) return 0;
     const size_t bytes = (size_t)capacity * 9 * c->plane * c->esz;
     HIP_TRY(hipMalloc(&c->slice_stage, bytes));
     HIP_TRY(hipHostMalloc(&c->slice_host, bytes, hipHostMallocDefault));
     if (!c->side) HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->slice_ev) HIP_TRY(hipEventCreateWithFlags(&c->slice_ev, hipEventDisableTiming));
+    c->slice_interval = interval;
+    c->slice_cap = capacity;
     return 0;
 }
 
@@ -577,13 +588,15 @@ extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity, int every)
 {
     if (!c || capacity < 0 || every < 1) return fdtd_fail(FDTD_EINVAL, "bad timing capacity or stride");
     HIP_TRY(hipDeviceSynchronize());
-    for (auto ev : c->tev) (void)hipEventDestroy(ev);
-    c->tev.assign(2 * (size_t)capacity, nullptr);
-    for (auto &ev : c->tev) HIP_TRY(hipEventCreate(&ev));
-    c->tcap = capacity;
+    for (auto ev : c->tev)
+        if (ev) (void)hipEventDestroy(ev);
+    c->tcap = 0;                       // stays 0 (no timed launches) unless every event is created
     c->tused = 0;
     c->tevery = every;
     c->tseen = 0;
+    c->tev.assign(2 * (size_t)capacity, nullptr);
+    for (auto &ev : c->tev) HIP_TRY(hipEventCreate(&ev));
+    c->tcap = capacity;
     return 0;
 }
 
@@ -695,15 +708,14 @@ extern "C" int fdtd_ctx_set_energy(fdtd_ctx *c, int interval, int capacity)
     HIP_TRY(hipDeviceSynchronize());
     (void)hipFree(c->energy_out);
     c->energy_out = nullptr;
-    c->energy_interval = interval;
-    c->energy_cap = capacity;
+    c->energy_interval = 0;            // stays 0 (no sums) unless the allocation succeeds
+    c->energy_cap = 0;
     c->energy_rows = 0;
     c->energy_steps.clear();
-    if (interval == 0 || capacity == 0) {
-        c->energy_interval = 0;
-        return 0;
-    }
+    if (interval == 0 || capacity == 0) return 0;
     HIP_TRY(hipMalloc(&c->energy_out, sizeof(double) * ((size_t)capacity + kEnergyBlocks)));
+    c->energy_interval = interval;
+    c->energy_cap = capacity;
     return 0;
 }
 
@@ -777,6 +789,17 @@ extern "C" int fdtd_ctx_load(fdtd_ctx *c, const char *path)
         h.storage != c->storage || h.esz != (int)c->esz) {
         fclose(f);
         return fdtd_fail(FDTD_ERANGE, "snapshot %s is of another grid or storage", path);
+    }
+    if (h.n < 0 || (h.hcur != 0 && h.hcur != 1) || !(h.sE > 0.0 && h.sD > 0.0 && h.sH > 0.0) ||
+        !std::isfinite(h.sE) || !std::isfinite(h.sD) || !std::isfinite(h.sH)) {
+        fclose(f);
+        return fdtd_fail(FDTD_EIO, "snapshot %s has a corrupt header", path);
+    }
+    // the whole file is checked before any device field changes
+    const long long want = (long long)sizeof h + 24LL * (long long)(c->Np * c->esz);
+    if (fseeko(f, 0, SEEK_END) != 0 || (long long)ftello(f) != want || fseeko(f, (off_t)sizeof h, SEEK_SET) != 0) {
+        fclose(f);
+        return fdtd_fail(FDTD_EIO, "snapshot %s is not %lld bytes long", path, want);
     }
     HIP_TRY(hipDeviceSynchronize());
     std::vector<char> buf(c->Np * c->esz);

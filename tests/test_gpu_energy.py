@@ -88,6 +88,18 @@ def test_snapshot_restart_is_bit_exact(tmp_path, storage):
     other = F.Context(F.make_params((36, 21, 18), DT, DX, mu0=MU0, eps0=EPS0), storage=storage)
     with pytest.raises(F.FdtdError, match="-2"):
         other.load(tmp_path / "state.bin")
+    # a corrupt header (H buffer index 7 at byte 44) or a cut-off file is refused, and the
+    # context keeps its state
+    raw = bytearray((tmp_path / "state.bin").read_bytes())
+    bad = bytearray(raw)
+    bad[44:48] = (7).to_bytes(4, "little")
+    (tmp_path / "bad.bin").write_bytes(bytes(bad))
+    (tmp_path / "cut.bin").write_bytes(bytes(raw[:len(raw) // 2]))
+    c = make()
+    for name in ("bad.bin", "cut.bin"):
+        with pytest.raises(F.FdtdError, match="-4"):
+            c.load(tmp_path / name)
+    assert c.n == 0
 
 
 @pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16])

@@ -49,6 +49,7 @@ extern "C" int fdtd_dropin_solve(const FdtdParams *p, const FdtdExt *ext, const 
     if (in->nmat < 1 || in->nmat > kMaxMaterials || !in->table || in->steps < 1) return fdtd_fail(FDTD_EINVAL, "bad grid inputs");
     if (in->nsrc < 0 || in->nsrc > D_MAX_SOURCES || (in->nsrc > 0 && !in->src_xyz))
         return fdtd_fail(FDTD_ERANGE, "the drop-in path takes at most %d sources", D_MAX_SOURCES);
+    if (in->jz_len < 0 || (in->jz_len > 0 && !in->jz)) return fdtd_fail(FDTD_EINVAL, "bad waveform");
     if (in->nprobe < 0 || (in->nprobe > 0 && (!in->probes || !probes_host))) return fdtd_fail(FDTD_EINVAL, "bad probes");
     if (int rc = fdtd_check_sources(nx, ny, nz, in->nsrc, in->src_xyz)) return rc;
     if (int rc = fdtd_check_probes(nx, ny, nz, in->nprobe, in->probes)) return rc;

@@ -57,7 +57,8 @@ struct Scanner {
         if (*c == '-' || *c == '+') neg = *c++ == '-';
         if (!isdigit((unsigned char)*c)) return false;
         long x = 0;
-        while (isdigit((unsigned char)*c)) x = x * 10 + (*c++ - '0');
+        for (; isdigit((unsigned char)*c); c++)
+            x = std::min(x * 10 + (*c - '0'), 1L << 40);   // saturates: every range check still fails
         v = neg ? -x : x;
         return true;
     }
@@ -75,9 +76,10 @@ struct Scanner {
             }
             if (!integer(v[q])) return false;
         }
+        if (v[0] < 1 || v[1] < 1 || v[0] > (1L << 30) || v[1] > (1L << 30)) return false;
         w = (int)v[0];
         h = (int)v[1];
-        return w > 0 && h > 0;
+        return true;
     }
 };
 

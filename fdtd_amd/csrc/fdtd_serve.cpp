@@ -269,10 +269,16 @@ void worker_main(fdtd_serve *s)
         while (!rc && (s->inflight.size() > 1 || (s->active.empty() && !s->inflight.empty()))) rc = retire(s);
         if (rc) {
             // A device error ends the server: every admitted and queued request fails.
+            (void)hipStreamSynchronize(s->stream);  // queued probe copies may still write the pinned buffers
             std::lock_guard<std::mutex> lk(s->mu);
-            for (Request *r : s->active) fail_request(s, r, rc);
+            auto drop = [&](Request *r) {
+                if (r->pinned) s->pinned_pool.emplace(r->pinned_bytes, r->pinned);  // freed with the server
+                r->pinned = nullptr;
+                fail_request(s, r, rc);
+            };
+            for (Request *r : s->active) drop(r);
             for (auto &seg : s->inflight) {
-                for (Request *r : seg.finished) fail_request(s, r, rc);
+                for (Request *r : seg.finished) drop(r);
                 s->spare_events.push_back(seg.ev);  // destroyed with the server
             }
             for (Request *r : s->queue) fail_request(s, r, rc);

@@ -159,6 +159,16 @@ def test_dropin_solve_equals_fused():
         got = F.dropin_solve(inst)
         want = single(inst)[1]
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), f"jz mode {jm}"
+    # rejected before any device work: a waveform length without its values, a material id
+    # past the table
+    bad = make_case(310, (20, 18, 16), 4, 5)
+    bad.c.jz = None
+    with pytest.raises(F.FdtdError, match="-1"):
+        F.dropin_solve(bad)
+    bad = make_case(311, (20, 18, 16), 4, 5)
+    bad.ids[3] = 4
+    with pytest.raises(F.FdtdError, match="-2"):
+        F.dropin_solve(bad)
 
 
 def test_batch_reference_debye_mode():

@@ -63,6 +63,15 @@ def test_errors(tmp_path):
         F.load_phantom_ids(tmp_path, (4, 4, 2), F.PGM_IMAGE)
     with pytest.raises(F.FdtdError, match="-4"):
         F.load_tissue_table(tmp_path / "no_such_table")
+    # a code or an image size past the range of any integer type is refused, not wrapped
+    (tmp_path / "v1_00002.pgm").write_text("P2\n4 4\n94\n" + "0 " * 15 + "9" * 40 + "\n")
+    with pytest.raises(F.FdtdError, match="-2"):
+        F.load_phantom_ids(tmp_path, (4, 4, 2), F.PGM_IMAGE)
+    with pytest.raises(F.FdtdError, match="-2"):
+        F.load_phantom_ids(tmp_path, (4, 4, 2), F.PGM_REFERENCE)
+    (tmp_path / "v1_00002.pgm").write_text("P2\n4294967297 1\n94\n0\n")
+    with pytest.raises(F.FdtdError, match="-4"):
+        F.load_phantom_ids(tmp_path, (4, 4, 2), F.PGM_IMAGE)
 
 
 def test_tissue_table_matches_oracle():

@@ -277,6 +277,12 @@ class Context:
 
     __del__ = close
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def set_materials(self, ids, table):
         ids = np.ascontiguousarray(ids, np.uint8).ravel()
         table = np.ascontiguousarray(table, np.float32).reshape(-1, 4)
@@ -518,6 +524,12 @@ class Batch:
 
     __del__ = close
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def add(self, inst):
         i = C.c_int()
         check(lib().fdtd_batch_add(self._h, C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c),
@@ -584,6 +596,12 @@ class Server:
             self._h = None
 
     __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def submit(self, inst):
         r = C.c_longlong()

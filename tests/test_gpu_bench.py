@@ -37,6 +37,16 @@ def test_bench_line_contract():
     assert d["config"]["workload"] == "plain_small" and d["config"]["grid"] == [64, 64, 64]
 
 
+def test_bench_16bit_line_has_its_ceiling():
+    d = _bench("--workload", "plain_small", "--storage", "f16", "--steps", "16", "--warmup", "2",
+               "--no-cpu-baseline", "--no-autotune")
+    r = d["roofline"]
+    assert r["bytes_per_cell"] == 49 and d["dtype"].startswith("f16")
+    c = r["stream_ceiling"]
+    assert c["gb_per_s"] > 0 and abs(c["frac"] - r["achieved"] / c["gb_per_s"]) < 1e-3
+    assert "131072 32-bit words" in c["what"]      # 64^3 cells, two per word
+
+
 def test_bench_slabs_box_phantom():
     d = _bench("--workload", "slabs", "--grid", "96x64x60", "--slabs", "3", "--steps", "12", "--warmup", "2")
     assert d["config"]["workload"] == "box_phantom_slabs" and d["config"]["slab_planes"] == [20, 20, 20]

@@ -3,15 +3,21 @@
 
 A "step" is one full FDTD time step of the grid: the fused H/D/source/Debye-E
 kernel, the Mur face kernel (which also records the probe row) and (every
-100 steps) the z-slice capture.  The default workload is the 256^3 eight-ellipsoid Debye
-phantom with a Jz sheet source at 900 MHz, 3 probes and a z-slice every 100
-steps (plain_256 in fdtd_amd/workloads.py), 2000 steps.
+500 steps) the z-slice capture.  The default workload is the 320x320x880
+voxel phantom of 50 Debye tissues with a 10-cell Jz dipole at 900 MHz, 3
+probes and a z-slice every 500 steps (large_phantom in
+fdtd_amd/workloads.py).  The grid first advances --advance steps untimed (so
+the timed steps run on a populated field), then --warmup steps, then the
+--steps timed steps; the line states the timed step range.
 
-Prints one JSON line: Gcell-updates/s over the timed steps, the fused
-kernel's roofline (97 algorithmic bytes per cell-update over its mean
-launch duration, timed with HIP events on its stream) and the CPU baseline
-(the oracle's FP32 transcription, OpenMP on this host's cores, on a bounded
-sample of the same workload).
+Prints one JSON line: Gcell-updates/s over the timed steps (HIP events on
+the bench's stream around the whole step sequence), the fused kernel's
+roofline (97 algorithmic bytes per cell-update over its mean launch
+duration, from HIP events bound to each dispatch over --timing-launches
+further steps, at least 64, taken after the timed steps so timing costs the
+timed steps nothing) and the CPU baseline (the oracle's FP32 transcription,
+OpenMP on this host's cores, on a bounded sample of the same workload).
+Exits with status 3 when the probes saw no finite non-zero signal.
 
 Two more workloads measure the systems patterns:
   --workload batched  64 uneven grids (batched_set) stepped together, one
@@ -29,6 +35,12 @@ Two more workloads measure the systems patterns:
   --workload stream   the memory ceiling of the fused step's access mix: 15
                       arrays read and 9 written per cell with no arithmetic
                       (fdtd_stream_probe), in GB/s.
+  --workload cli      the `fdtd` command-line driver on the reference's own
+                      parameter file (data/input_params: 100^3, 3 GHz, 1 mm
+                      cells) with synthetic tissue slices, --steps steps and all
+                      nine fields dumped as text after every step: wall time of
+                      the overlapped result pipeline beside --sync (each dump
+                      written on the stepping thread).
   --workload slabs    320x320x880 cut into --slabs z-slabs (each with ghost
                       planes), all stepped on this one GPU with the halo
                       exchange of every interface before every step: the
@@ -55,11 +67,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--time-every", type=int, default=16,
-                    help="time every N-th fused launch of the timed steps (HIP events bound to the dispatch)")
-    ap.add_argument("--workload", default="plain_256",
+    ap.add_argument("--advance", type=int, default=300,
+                    help="untimed steps before the warm-up, so the timed steps run on a populated field")
+    ap.add_argument("--timing-launches", type=int, default=64,
+                    help="fused launches timed one by one (HIP events bound to each dispatch) after the timed "
+                         "steps, for roofline.achieved; at least 64")
+    ap.add_argument("--workload", default="large_phantom",
                     choices=["plain_256", "large_phantom", "plain_small", "batched", "served", "riken", "slabs",
-                             "stream"])
+                             "stream", "cli"])
     ap.add_argument("--slabs", type=int, default=8, help="slabs: z-slabs of the 320x320x880 grid")
     ap.add_argument("--grid", default=None,
                     help="slabs: NXxNYxNZ of a box phantom in place of the 320x320x880 grid (any size that "
@@ -78,10 +93,14 @@ def parse():
     ap.add_argument("--no-sequential", action="store_true", help="batched/served: skip the sequential baselines")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="field storage (f32 reproduces the reference's arithmetic bit for bit)")
-    ap.add_argument("--cpu-steps", type=int, default=300,
-                    help="steps of the CPU baseline sample (about 10 s of host work at 256^3)")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the CPU baseline sample (0: about 5 G cell-updates, 10-15 s of host work on "
+                         "16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shape by timing each candidate first (default: the library's "
+                         "default shape, 64x4 tiles of 4 planes, so the command's profile is reproducible)")
+    ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape TXxTYxKC (threads per tile row, tile rows, z planes per block), "
                          "e.g. 128x2x6; no autotuning")
@@ -106,26 +125,30 @@ def build_workload(name, steps, a=None):
     return W.plain_small(steps=steps)
 
 
-PROFILE = os.path.join(ROOT, "profiles", "fused_step_plain256_large_phantom.md")
+PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phantom.md",
+                                                          "fused_step_plain256_large_phantom.md")]
 
 
 def profiled_traffic(workload, storage, launch):
-    """HBM bytes per fused launch measured for this workload at this launch
-    shape (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB
-    copy) in the committed profile, or None."""
-    if storage != "f32" or workload not in ("plain_256", "large_phantom") or not os.path.exists(PROFILE):
-        return None
+    """(HBM bytes per fused launch, profile path) measured for this workload
+    at this launch shape (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes,
+    scaled by a 1 GiB copy) in a committed profile, or (None, None)."""
     import re
-    text = open(PROFILE).read()
+    if storage != "f32":
+        return None, None
     head = f"## {workload} at {launch}:"
-    if head not in text:
-        return None
-    sec = text.split(head, 1)[1].split("\n## ", 1)[0]
-    rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
-    wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
-    if not rd or not wr:
-        return None
-    return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024
+    for path in PROFILES:
+        if not os.path.exists(path):
+            continue
+        text = open(path).read()
+        if head not in text:
+            continue
+        sec = text.split(head, 1)[1].split("\n## ", 1)[0]
+        rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
+        wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
+        if rd and wr:
+            return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024, path
+    return None, None
 
 
 def host_cpu():
@@ -194,8 +217,11 @@ def main():
         return bench_slabs(a)
     if a.workload == "stream":
         return bench_stream(a)
+    if a.workload == "cli":
+        return bench_cli(a)
 
-    total = a.warmup + a.steps
+    timing = max(64, a.timing_launches)
+    total = a.advance + a.warmup + a.steps + timing
     w = build_workload(a.workload, total, a)
     nx, ny, nz = w.shape
     cells = nx * ny * nz
@@ -214,12 +240,12 @@ def main():
     if w.slice_every:
         ctx.set_slices(w.slice_k, w.slice_every, total // w.slice_every + 1)
     tuned = None
-    if not (a.no_autotune or a.shape):
+    if a.autotune and not a.shape:
         # untimed: the autotuner runs each launch shape a few steps, then resets the grid to step 0
         tuned = ctx.autotune(trial_steps=8, stream=stream.handle)
-    ctx.run(a.warmup, stream.handle)
+    ctx.run(a.advance + a.warmup, stream.handle)
     stream.sync()
-    ctx.set_timing(a.steps, every=max(1, a.time_every))
+    first = ctx.n
 
     e0, e1 = hip.Event(), hip.Event()
     hip.sync()
@@ -229,9 +255,13 @@ def main():
     e1.sync()
     hip.sync()
     ms = e0.elapsed_ms(e1)
+    # the fused kernel's own duration: every launch of `timing` further steps, events bound to the dispatch
+    ctx.set_timing(timing, every=1)
+    ctx.run(timing, stream.handle)
+    stream.sync()
     kms, launches = ctx.kernel_time()
     probes = ctx.probes()
-    ok = bool(abs(probes).max() > 0) and bool((probes == probes).all())
+    ok = bool(np.isfinite(probes).all()) and bool(np.abs(probes).max() > 0)
 
     value = cells * a.steps / (ms * 1e-3) / 1e9
     kavg_s = kms / launches * 1e-3
@@ -242,10 +272,10 @@ def main():
     if a.traffic_kib:
         traffic, traffic_src = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024), "--traffic-kib"
     else:
-        traffic = profiled_traffic(w.name, a.storage, launch)
-        traffic_src = (os.path.relpath(PROFILE, ROOT) + f" (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
+        traffic, prof = profiled_traffic(w.name, a.storage, launch)
+        traffic_src = (os.path.relpath(prof, ROOT) + f" (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
                        f"scale; {launch} launch)") if traffic is not None else \
-            f"no counter profile of {w.name} at launch {launch} in {os.path.relpath(PROFILE, ROOT)}"
+            f"no counter profile of {w.name} at launch {launch} under profiles/"
     out = {
         "metric": "Gcell-updates/s (fused FDTD step, Debye media, Mur ABC)",
         "value": round(value, 3),
@@ -263,11 +293,13 @@ def main():
         "config": {"workload": w.name, "grid": [nx, ny, nz], "cells": cells, "materials": int(w.table.shape[0]),
                    "sources": int(w.sources.shape[0]), "probes": int(w.probes.shape[0]),
                    "slice_every": w.slice_every, "launch": shape, "autotuned": tuned is not None,
+                   "timed_steps": [first, first + a.steps], "untimed_advance": a.advance,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "fused_step", "kernel_avg_ms": round(kms / launches, 5),
                      "launches_timed": launches,
+                     "timed_launch_steps": [first + a.steps, first + a.steps + launches],
                      "bytes_per_cell": bpc},
         "probes_finite_nonzero": ok,
     }
@@ -284,30 +316,98 @@ def main():
             "one fp32 word per cell" if a.storage == "f32" else f"{words} 32-bit words (two 16-bit cells each)")
         + "; best of the resident grid, 8192 and 16384 blocks"}
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps or max(2, int(round(5e9 / cells))))
+    print(json.dumps(out), flush=True)
+    if not ok:
+        print("bench: the probes saw no finite non-zero signal", file=sys.stderr, flush=True)
+        sys.exit(3)
+
+
+def bench_cli(a):
+    """The CLI with a dump of all nine fields after every step, overlapped
+    output against --sync, on the reference's 100^3 parameter file."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "fdtd_amd", "fdtd")
+    specs = os.path.join(ROOT, "tests", "golden", "mat_specs_riken")
+    listed = [int(l.split()[0]) for l in open(specs) if l[:1].isdigit()]
+    steps = a.steps if a.steps % 3 == 0 else a.steps + 3 - a.steps % 3   # the parameter file rounds up to 3
+    work = tempfile.mkdtemp(prefix="fdtd_cli_")
+    try:
+        pgm = os.path.join(work, "pgm")
+        os.mkdir(pgm)
+        rng = np.random.default_rng(12352)
+        for k in range(100):
+            vals = rng.choice(listed, 100 * 100)
+            with open(os.path.join(pgm, f"v1_{k + 1:05d}.pgm"), "w") as f:
+                f.write("P2\n# test\n100 100\n94\n" + "\n".join(str(int(v)) for v in vals) + "\n")
+        text = open(os.path.join(ROOT, "tests", "golden", "input_params")).read()
+        text = text.replace("t_max: 3", f"t_max: {steps}").replace("env_set_dir: data/pgmdata",
+                                                                   f"env_set_dir: {pgm}")
+        runs = {}
+        for mode, extra in (("sync", ["--sync"]), ("overlapped", [])):
+            out = os.path.join(work, "out_" + mode)
+            os.mkdir(out)
+            params = os.path.join(work, "params_" + mode)
+            with open(params, "w") as f:
+                f.write(text.replace("output_dir: output", f"output_dir: {out}"))
+            t0 = time.perf_counter()
+            r = subprocess.run([cli, params, "--specs", specs, "--quiet", "--every", "1", *extra],
+                               capture_output=True, text=True, env=dict(os.environ, FDTD_TIMING="1"), timeout=1200)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"fdtd {mode}: {r.stdout[-2000:]}")
+            m = re.search(r"Stepping and output: ([\d.]+) s; the stepping thread waited ([\d.]+) s", r.stdout)
+            files = os.listdir(out)
+            runs[mode] = {"process_s": round(wall, 3), "stepping_and_output_s": float(m.group(1)),
+                          "stepping_thread_waited_s": float(m.group(2)), "files": len(files),
+                          "bytes": sum(os.path.getsize(os.path.join(out, n)) for n in files)}
+            print(f"[cli] {mode}: {runs[mode]}", file=sys.stderr, flush=True)
+        same = all(open(os.path.join(work, "out_sync", n), "rb").read() ==
+                   open(os.path.join(work, "out_overlapped", n), "rb").read()
+                   for n in sorted(os.listdir(os.path.join(work, "out_sync"))))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    t_sync, t_ovl = runs["sync"]["stepping_and_output_s"], runs["overlapped"]["stepping_and_output_s"]
+    out = {
+        "metric": "dumps/s (CLI, all nine fields as text after every step)",
+        "value": round(steps / t_ovl, 3), "unit": "dumps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(t_ovl / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic tissue slices (codes of data/mat_specs_riken)",
+        "config": {"workload": "cli_input_params", "grid": [100, 100, 100], "every": 1, "writers": 7,
+                   "parallelism": "single GPU"},
+        "runs": runs, "speedup_vs_sync": round(t_sync / t_ovl, 3), "outputs_identical": same,
+    }
     print(json.dumps(out), flush=True)
 
 
 def bench_stream(a):
     """The memory ceiling of the fused step's access mix: 15 arrays read and
-    9 written per cell (256^3 floats each), no arithmetic, every access
-    nontemporal (fdtd_stream_probe), at the resident block count and at two
-    larger grids."""
+    9 written per cell, no arithmetic, every access nontemporal
+    (fdtd_stream_probe_layout), at 256^3 and 320x320x880 cells, in two
+    layouts (24 separate arrays as a context holds them, and row records:
+    the 24 rows of one grid row contiguous), each at the resident block count
+    and two larger grids."""
     import fdtd_amd as F
-    cells = 256 ** 3
+    reps = max(2, a.steps // 100)
     runs = {}
-    for blocks in (0, 4096, 8192):
-        gbs, ms = F.stream_probe(cells, blocks, reps=max(1, a.steps // 100))
-        runs["resident" if blocks == 0 else str(blocks)] = {"gb_per_s": round(gbs, 1), "ms_per_pass": round(ms, 5)}
+    for name, cells, row in (("256^3", 256 ** 3, 256), ("320x320x880", 320 * 320 * 880, 320)):
+        for layout, w in (("arrays", 0), ("row_records", row)):
+            for blocks in (0, 8192, 16384):
+                gbs, ms = F.stream_probe_layout(cells, w, blocks, reps=reps)
+                runs[f"{name} {layout} {'resident' if blocks == 0 else blocks}"] = {
+                    "gb_per_s": round(gbs, 1), "ms_per_pass": round(ms, 5)}
+                print(f"[stream] {name} {layout} blocks={blocks}: {gbs:.1f} GB/s", file=sys.stderr, flush=True)
     best = max(v["gb_per_s"] for v in runs.values())
     out = {
         "metric": "GB/s streaming the fused step's 15-read / 9-write access mix without arithmetic",
-        "value": best, "unit": "GB/s", "n_gpus": 1, "steps": max(1, a.steps // 100), "warmup": 1,
+        "value": best, "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (zeros)",
-        "config": {"workload": "stream_mix", "cells": cells, "arrays": {"read": 15, "written": 9},
-                   "parallelism": "single GPU"},
-        "by_grid": runs, "frac_of_peak": round(best / HBM_PEAK_GBS, 4),
+        "config": {"workload": "stream_mix", "arrays": {"read": 15, "written": 9}, "parallelism": "single GPU"},
+        "by_layout": runs, "frac_of_peak": round(best / HBM_PEAK_GBS, 4),
     }
     print(json.dumps(out), flush=True)
 
@@ -479,7 +579,9 @@ def bench_batched(a):
     ids = [b.add(x) for x in insts]
     t_upload = time.perf_counter() - t0
     stream = hip.Stream()
+    t0 = time.perf_counter()
     ty, kc = b.autotune(trial_steps=4, stream=stream.handle)
+    t_tune = time.perf_counter() - t0
     info = b.info()
     steps = max(w.steps for w in ws)
     e0, e1 = hip.Event(), hip.Event()
@@ -507,8 +609,11 @@ def bench_batched(a):
                      "unit": "GB/s", "frac": round(97 * work / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "whole batch step (fused_batch + mur_batch with probe rows)",
                      "bytes_per_cell": 97},
-        "upload_s": round(t_upload, 3), "all_finished": ok,
+        "upload_s": round(t_upload, 3), "autotune_s": round(t_tune, 3), "all_finished": ok,
     }
+    # end to end, as the sequential runs below are timed: uploads, the autotuner and the run
+    e2e = t_upload + t_tune + ms * 1e-3
+    out["end_to_end"] = {"seconds": round(e2e, 3), "gcell_per_s": round(work / e2e / 1e9, 3)}
     if a.storage != "f32":
         out["roofline"]["bytes_per_cell"] = BYTES_PER_CELL[a.storage]
         out["roofline"]["achieved"] = round(BYTES_PER_CELL[a.storage] * work / (ms * 1e-3) / 1e9, 1)
@@ -518,8 +623,11 @@ def bench_batched(a):
         td = _sequential_dropin(F, insts)
         out["sequential_fused"] = {"seconds": round(sum(tf), 3), "gcell_per_s": round(work / sum(tf) / 1e9, 3)}
         out["sequential_dropin"] = {"seconds": round(sum(td), 3), "gcell_per_s": round(work / sum(td) / 1e9, 3)}
-        out["speedup_vs_sequential_dropin"] = round(sum(td) / (ms * 1e-3), 3)
-        out["speedup_vs_sequential_fused"] = round(sum(tf) / (ms * 1e-3), 3)
+        # both sides end to end: each sequential solve includes its set-up, upload and free, the batch its
+        # uploads and autotuning
+        out["speedup_vs_sequential_dropin"] = round(sum(td) / e2e, 3)
+        out["speedup_vs_sequential_fused"] = round(sum(tf) / e2e, 3)
+        out["speedup_basis"] = "end to end on both sides (set-up and uploads included)"
     print(json.dumps(out), flush=True)
 
 

@@ -45,14 +45,14 @@ EXPORTS = {
                    "fdtd_ctx_set_materials", "fdtd_ctx_set_material_arrays", "fdtd_ctx_set_rp",
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
-                   "fdtd_ctx_steps_done", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
+                   "fdtd_ctx_steps_done", "fdtd_ctx_snapshot", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
                    "fdtd_ctx_autotune", "fdtd_ctx_halo_bytes", "fdtd_ctx_halo_pack", "fdtd_ctx_halo_unpack",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
-                   "fdtd_dropin_solve", "fdtd_stream_probe", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
+                   "fdtd_dropin_solve", "fdtd_stream_probe", "fdtd_stream_probe_layout", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
@@ -163,6 +163,7 @@ def lib():
         L.fdtd_step_fused.argtypes = [vp, vp]
         L.fdtd_run.argtypes = [vp, i, vp]
         L.fdtd_ctx_steps_done.argtypes = [vp]
+        L.fdtd_ctx_snapshot.argtypes = [vp, vp, vp]
         L.fdtd_probe_read.argtypes = [vp, vp, i]
         L.fdtd_slices_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
@@ -193,6 +194,7 @@ def lib():
         L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
         L.fdtd_dropin_solve.argtypes = [PP, PE, PI, vp, vp]
         L.fdtd_stream_probe.argtypes = [C.c_longlong, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.fdtd_stream_probe_layout.argtypes = [C.c_longlong, i, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fdtd_load_tissue_table.argtypes = [C.c_char_p, vp]
         L.fdtd_load_phantom_ids.argtypes = [C.c_char_p, i, i, i, i, i, vp]
         L.fdtd_serve_create.argtypes = [C.POINTER(FdtdServeConfig), C.POINTER(vp)]
@@ -369,6 +371,11 @@ class Context:
             check(got, "probe_read")
         return out[:got, :self.nprobe]
 
+    def snapshot(self, dev_out, stream=None):
+        """Enqueue the nine newest fields, packed fp32 in SI units, into the
+        device buffer at address dev_out (9*nx*ny*nz floats)."""
+        check(lib().fdtd_ctx_snapshot(self._h, dev_out, stream), "snapshot")
+
     def slices(self, max_slices=1024):
         nx, ny, _ = self.shape
         steps = np.zeros(max_slices, np.int32)
@@ -477,6 +484,15 @@ def stream_probe(cells, blocks=0, reps=20):
     (fdtd_stream_probe); blocks 0 = as many as fit resident."""
     g, ms = C.c_double(), C.c_double()
     check(lib().fdtd_stream_probe(int(cells), int(blocks), int(reps), C.byref(g), C.byref(ms)), "fdtd_stream_probe")
+    return g.value, ms.value
+
+
+def stream_probe_layout(cells, row, blocks=0, reps=20):
+    """(GB/s, ms per pass) of the same mix with the 24 arrays interleaved by
+    rows of `row` elements (row records; 0 = separate arrays)."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().fdtd_stream_probe_layout(int(cells), int(row), int(blocks), int(reps), C.byref(g), C.byref(ms)),
+          "fdtd_stream_probe_layout")
     return g.value, ms.value
 
 

@@ -260,8 +260,10 @@ int fdtd_check_sources(int nx, int ny, int nz, int nsrc, const int *xyz)
 {
     for (int s = 0; s < nsrc; s++) {
         const int x = xyz[3 * s], y = xyz[3 * s + 1], z = xyz[3 * s + 2];
-        if (x < 1 || x >= nx - 1 || y < 1 || y >= ny - 1 || z < 0 || z >= nz - 1)
-            return fdtd_fail(FDTD_ERANGE, "source %d at (%d,%d,%d) outside the Dz update range", s, x, y, z);
+        // updateSources reads Hy at x-1 and Hx at y-1 (fdtd_calculations.cu:205-208), so x and y
+        // start at 1; the upper faces and the top plane are allowed, as the reference writes Dz there.
+        if (x < 1 || x >= nx || y < 1 || y >= ny || z < 0 || z >= nz)
+            return fdtd_fail(FDTD_ERANGE, "source %d at (%d,%d,%d) outside 1<=x<nx, 1<=y<ny, 0<=z<nz", s, x, y, z);
     }
     return 0;
 }
@@ -322,23 +324,28 @@ void mur_x_coefs(int ny, int nz, float dt, float hx, float mu0, float eps0, cons
         }
 }
 
-extern "C" int fdtd_stream_probe(long long cells, int blocks, int reps, double *gbs, double *ms_per_pass)
+extern "C" int fdtd_stream_probe_layout(long long cells, int row, int blocks, int reps, double *gbs,
+                                        double *ms_per_pass)
 {
-    if (cells < 1 || reps < 1 || !gbs) return fdtd_fail(FDTD_EINVAL, "stream probe: cells %lld, reps %d", cells, reps);
+    if (cells < 1 || reps < 1 || !gbs || row < 0)
+        return fdtd_fail(FDTD_EINVAL, "stream probe: cells %lld, row %d, reps %d", cells, row, reps);
+    if (row > 0 && cells % row != 0) return fdtd_fail(FDTD_EINVAL, "stream probe: cells %lld not whole rows of %d", cells, row);
     if (blocks <= 0) blocks = stream_mix_resident_blocks();
     if (blocks <= 0) return fdtd_fail(FDTD_EINVAL, "stream probe: no resident blocks");
-    const size_t stride = (size_t)cells + 65536;  // the contexts' stagger between arrays
+    // separate arrays: the contexts' stagger between arrays; row records: one block
+    const size_t stride = row > 0 ? (size_t)cells : (size_t)cells + 65536;
     float *blob = nullptr;
     HIP_TRY(hipMalloc(&blob, 24 * stride * sizeof(float)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.0f;
     StreamMix p;
     for (int a = 0; a < 24; a++) {
-        float *b = blob + a * stride;
+        float *b = blob + (row > 0 ? 0 : a * stride);
         if (a < 15) p.r[a] = b;
         else p.w[a - 15] = b;
     }
     p.n = cells;
+    p.row = row;
     hipError_t e = hipMemset(blob, 0, 24 * stride * sizeof(float));
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -356,4 +363,9 @@ extern "C" int fdtd_stream_probe(long long cells, int blocks, int reps, double *
     *gbs = 24.0 * sizeof(float) * (double)cells / (per * 1e-3) / 1e9;
     if (ms_per_pass) *ms_per_pass = per;
     return 0;
+}
+
+extern "C" int fdtd_stream_probe(long long cells, int blocks, int reps, double *gbs, double *ms_per_pass)
+{
+    return fdtd_stream_probe_layout(cells, 0, blocks, reps, gbs, ms_per_pass);
 }

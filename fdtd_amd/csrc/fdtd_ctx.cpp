@@ -111,6 +111,8 @@ int upload_ids(fdtd_ctx *c)
     }
     std::vector<unsigned char> img(c->Np);
     fdtd_pitch_bytes(merged.data(), c->nx, (size_t)c->ny * c->nz, c->pitch, img.data());
+    // steps queued on any stream (a non-blocking one included) may still read the ids
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(c->ids, img.data(), c->Np, hipMemcpyHostToDevice));
     return 0;
 }
@@ -130,6 +132,8 @@ int upload_lut(fdtd_ctx *c)
     if (!c->lut_dirty) return 0;
     std::vector<MatCoef> lut(c->nmat);
     fdtd_build_lut(c->table.data(), c->nmat, c->dt, c->eps0, c->ext.debye_mode, c->storage, c->sE, c->sD, lut.data());
+    // only after set_materials or the first step: queued steps may still read the old table
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(c->lut, lut.data(), sizeof(MatCoef) * c->nmat, hipMemcpyHostToDevice));
     c->lut_dirty = false;
     return 0;
@@ -329,10 +333,12 @@ extern "C" int fdtd_ctx_set_rp(fdtd_ctx *c, const float *rpx0, const float *rpy0
     const float *src[6] = {rpx0, rpy0, rpz0, rpxEnd, rpyEnd, rpzEnd};
     const size_t rps[6] = {(size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2,
                            (size_t)2 * c->ny * c->nz, (size_t)c->nx * 2 * c->nz, (size_t)c->nx * c->ny * 2};
-    for (int r = 0; r < 6; r++) {
+    for (int r = 0; r < 6; r++)
         if (!src[r]) return fdtd_fail(FDTD_EINVAL, "null rp plane %d", r);
+    // queued steps (on any stream) may still read the planes and the x-face coefficients
+    HIP_TRY(hipDeviceSynchronize());
+    for (int r = 0; r < 6; r++)
         HIP_TRY(hipMemcpy(c->rp[r], src[r], rps[r] * sizeof(float), hipMemcpyHostToDevice));
-    }
     HIP_TRY(upload_murx(c, rpx0, rpxEnd));
     return 0;
 }
@@ -352,6 +358,7 @@ extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, i
 {
     if (!c || nprobe < 0 || capacity < 0 || (nprobe > 0 && !probes)) return fdtd_fail(FDTD_EINVAL, "bad probes");
     if (int rc = fdtd_check_probes(c->nx, c->ny, c->nz, nprobe, probes)) return rc;
+    HIP_TRY(hipDeviceSynchronize());   // queued Mur launches may still record into the old buffers
     (void)hipFree(c->probes);
     (void)hipFree(c->probe_out);
     c->probes = nullptr;
@@ -572,6 +579,21 @@ extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_sl
                                       c->nx, field_factor(c, f), data + ((size_t)s * 9 + f) * nxy + (size_t)j * c->nx);
     }
     return cnt;
+}
+
+extern "C" int fdtd_ctx_snapshot(fdtd_ctx *c, float *dev_out, hipStream_t stream)
+{
+    if (!c || !dev_out) return fdtd_fail(FDTD_EINVAL, "null argument");
+    resolve_scales(c);
+    SnapArgs a;
+    for (int f = 0; f < 9; f++) {
+        a.field[f] = field_dev(c, f, 0);
+        a.factor[f] = field_factor(c, f);
+    }
+    a.out = dev_out;
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.pitch = c->pitch; a.plane = c->plane;
+    const hipError_t e = launch_snapshot(a, c->storage, stream);
+    return e == hipSuccess ? 0 : fdtd_hip_fail(e, "snapshot");
 }
 
 extern "C" int fdtd_ctx_launch_shape(fdtd_ctx *c, int *tile_x, int *tile_y, int *kchunk, int *blocks)

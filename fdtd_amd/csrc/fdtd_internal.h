@@ -157,6 +157,10 @@ struct StreamMix {
     const float *r[15];
     float *w[9];
     long long n;
+    // row > 0: the 24 arrays interleaved by rows of `row` elements (row
+    // records: the 24 rows of cells [q*row, (q+1)*row) contiguous, 15 read
+    // then 9 written) from r[0]; r and w unused otherwise
+    int row;
 };
 hipError_t launch_stream_mix(const StreamMix &p, int blocks, hipStream_t s);
 // Blocks of the streaming probe resident on the whole device at once.
@@ -176,6 +180,18 @@ struct PlaneCopy {
     long long words;
 };
 hipError_t launch_plane_copy(const PlaneCopy &p, hipStream_t s);
+// The nine newest fields of a context as packed fp32 volumes in the
+// reference's layout (field f at out + f*nx*ny*nz, OFFSET order), stored
+// values times factor[f] in double (scales undone), as get_field computes
+// them on the host; fp32 storage copies the bits.
+struct SnapArgs {
+    const void *field[9];
+    double factor[9];
+    float *out;
+    int nx, ny, nz, pitch;
+    long long plane;
+};
+hipError_t launch_snapshot(const SnapArgs &a, int storage, hipStream_t s);
 // Mur faces and one probe row in one launch (p.nprobe = 0: faces only).  The
 // probe fields must be the newest levels, with E the Mur target a.exT..ezT.
 hipError_t launch_mur_probes(const MurArgs &a, const ProbeArgs &p, int storage, hipStream_t s);
@@ -331,6 +347,26 @@ int phantom_read_table(const char *path, float *specs, std::string &err);
 int phantom_read_slice(const char *path, int nx, int ny, int mode, unsigned char *ids, std::vector<char> &buf,
                        std::string &err);
 int fdtd_check_sources(int nx, int ny, int nz, int nsrc, const int *xyz);
+
+// Device state of one grid on the drop-in path (the reference's FdtdField
+// arrays, fdtd.h:37-54) and its probe buffers; shared by fdtd_dropin_solve
+// (which allocates it) and fdtd_dropin_solve_noalloc (which carves it from a
+// static device array).
+struct DropinState {
+    float *E[3][3], *D[3][3], *H[3];
+    float *mat[4];                 // sigma, eps_s, eps_inf, tau per cell
+    float *rp[6];
+    int *probes = nullptr;         // nprobe x (field, i, j, k)
+    double *probe_out = nullptr;   // steps x nprobe
+};
+// Argument checks of the drop-in solve; fills the effective FdtdExt.
+int dropin_check(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *in, const double *probes_host,
+                 FdtdExt *out);
+// Uploads the symbols, runs inst->steps steps of the five entry points on
+// `stream` over state whose fields start at zero, copies the probe rows to
+// probes_host and waits for the stream.
+int dropin_run(const FdtdParams *p, const FdtdExt &e, const FdtdInstance *in, const DropinState &st,
+               double *probes_host, hipStream_t stream);
 int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes);
 
 #define HIP_TRY(call)                                              \

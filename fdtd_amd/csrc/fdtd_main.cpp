@@ -14,11 +14,21 @@
 // drop-in entry points on the reference's FdtdField layout; the default runs
 // the fused context path.  The Debye update defaults to the stable form
 // (d = a + b - sigma dt^2); --debye ref selects fdtd_calculations.cu:144.
+//
+// Result output overlaps the steps, as the reference's copy and writer
+// threads do (fdtd.cu:17-43, 134-155, 253, 1136-1194); see Pipeline below.
+// --writers N bounds the dumps being written at once (default 7, the
+// reference's MAX_COPY_THREADS); --sync writes each dump on the stepping
+// thread before the next step instead.
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,12 +68,14 @@ struct Options {
     int every = 1;
     bool lines = false;
     bool quiet = false;
+    bool sync = false;
+    int writers = 7;
 };
 
 [[noreturn]] void usage()
 {
     printf("usage: fdtd [params] [--specs FILE] [--legacy] [--debye fixed|ref] [--jz hold3|step] "
-           "[--jz-limit N] [--every K] [--lines] [--quiet]\n");
+           "[--jz-limit N] [--every K] [--lines] [--quiet] [--writers N] [--sync]\n");
     exit(EXIT_FAILURE);
 }
 
@@ -81,10 +93,13 @@ Options parse(int argc, char **argv)
         else if (s == "--every") o.every = atoi(next().c_str());
         else if (s == "--lines") o.lines = true;
         else if (s == "--quiet") o.quiet = true;
+        else if (s == "--sync") o.sync = true;
+        else if (s == "--writers") o.writers = atoi(next().c_str());
         else if (s.size() > 1 && s[0] == '-') usage();
         else o.params = s;
     }
     if (o.every < 1) o.every = 1;
+    if (o.writers < 1) o.writers = 1;
     return o;
 }
 
@@ -97,6 +112,185 @@ void write_set(FdtdParams *p, float **f, int iteration, bool lines)
     else
         writeResults3d(p, f[3], f[4], f[5], f[0], f[1], f[2], f[6], f[7], f[8], iteration);
 }
+
+// writeResultsWithParams (fdtd.cu:1163-1194) with the line writer: waits for
+// the three copy threads, writes, frees what they allocated.
+void *write_lines_with_params(void *arg)
+{
+    ResultsParams *rp = (ResultsParams *)arg;
+    CopyParams *c[3] = {rp->hParams, rp->dParams, rp->eParams};
+    for (CopyParams *q : c) pthread_mutex_lock(q->copyMutex);
+    writeResults(rp->params, c[0]->xBuffer, c[0]->yBuffer, c[0]->zBuffer, c[1]->xBuffer, c[1]->yBuffer,
+                 c[1]->zBuffer, c[2]->xBuffer, c[2]->yBuffer, c[2]->zBuffer, rp->currentIteration);
+    for (CopyParams *q : c) {
+        pthread_mutex_unlock(q->copyMutex);
+        free(q->copyMutex);
+        free(q);
+    }
+    free(rp);
+    return nullptr;
+}
+
+// Result output overlapped with the steps.  Per dump:
+//   1. the nine newest fields are copied on the compute stream into a device
+//      staging set (fdtd_ctx_snapshot, or nine device copies on the legacy
+//      path), so later steps may overwrite the ring slots at once;
+//   2. the set's own side stream waits for that copy and moves the set to
+//      pinned host memory in one transfer;
+//   3. three copy threads (H, D, E) run copyResultsWithParams: each waits for
+//      the side stream and copies its three volumes out of pinned memory
+//      into malloc'ed buffers, after which the set can be reused;
+//   4. a writer thread runs writeResultsWithParams (or the line writer) on
+//      those buffers, which frees them.
+// At most `writers` dumps are in flight (the reference's MAX_COPY_THREADS
+// throttle, fdtd.cu:17-43, here a condition variable instead of a spin);
+// the stepping thread waits only for that throttle or for a staging set
+// still being copied out.  The copy mutexes are locked here and unlocked by
+// the copy threads, as the reference's pipeline does (fdtd.cu:151-155,
+// 1159): glibc's default mutex allows that.
+class Pipeline {
+public:
+    Pipeline(FdtdParams *p, bool lines, int nsets, int writers)
+        : p_(p), lines_(lines), writers_max_(writers), n_((size_t)p->nx * p->ny * p->nz)
+    {
+        sets_.resize(nsets);
+        for (Set &s : sets_) {
+            CHECK(hipMalloc(&s.dev, 9 * n_ * sizeof(float)));
+            CHECK(hipHostMalloc(&s.pinned, 9 * n_ * sizeof(float), hipHostMallocDefault));
+            CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+            CHECK(hipEventCreateWithFlags(&s.ready, hipEventDisableTiming));
+        }
+    }
+
+    ~Pipeline()
+    {
+        drain();
+        for (Set &s : sets_) {
+            (void)hipFree(s.dev);
+            (void)hipHostFree(s.pinned);
+            (void)hipStreamDestroy(s.stream);
+            (void)hipEventDestroy(s.ready);
+        }
+    }
+
+    // The device staging buffer of the next dump (9 volumes, field order Dx
+    // Dy Dz Hx Hy Hz Ex Ey Ez), once its previous contents are out and a
+    // writer slot is free.
+    float *begin()
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        Set &s = sets_[next_];
+        join_copies(s);
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [&] { return writers_ < writers_max_; });
+            writers_++;
+        }
+        wait_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return s.dev;
+    }
+
+    // Queues the transfer of the set begin() returned, after the work queued
+    // on `compute` so far, and starts its copy and writer threads.
+    void commit(hipStream_t compute, int iteration)
+    {
+        Set &s = sets_[next_];
+        next_ = (next_ + 1) % (int)sets_.size();
+        CHECK(hipEventRecord(s.ready, compute));
+        CHECK(hipStreamWaitEvent(s.stream, s.ready, 0));
+        CHECK(hipMemcpyAsync(s.pinned, s.dev, 9 * n_ * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+        CopyParams *cp[3];
+        for (int g = 0; g < 3; g++) {  // H, D, E: fields 3-5, 0-2, 6-8 of the set
+            const int f0 = g == 0 ? 3 : (g == 1 ? 0 : 6);
+            cp[g] = (CopyParams *)calloc(1, sizeof(CopyParams));
+            if (!cp[g]) { printf("Couldn't allocate copy parameters\n"); exit(EXIT_FAILURE); }
+            cp[g]->xSource = s.pinned + (size_t)f0 * n_;
+            cp[g]->ySource = s.pinned + (size_t)(f0 + 1) * n_;
+            cp[g]->zSource = s.pinned + (size_t)(f0 + 2) * n_;
+            cp[g]->params = p_;
+            cp[g]->stream = s.stream;
+            cp[g]->copyMutex = (pthread_mutex_t *)malloc(sizeof(pthread_mutex_t));
+            if (!cp[g]->copyMutex) { printf("Couldn't allocate a mutex\n"); exit(EXIT_FAILURE); }
+            pthread_mutex_init(cp[g]->copyMutex, nullptr);
+            pthread_mutex_lock(cp[g]->copyMutex);
+            if (pthread_create(&s.copier[g], nullptr, copyResultsWithParams, cp[g]) != 0) {
+                printf("Couldn't start a copy thread\n");
+                exit(EXIT_FAILURE);
+            }
+        }
+        s.copying = true;
+        ResultsParams *rp = (ResultsParams *)calloc(1, sizeof(ResultsParams));
+        Job *job = new Job{this, rp};
+        if (!rp) { printf("Couldn't allocate result parameters\n"); exit(EXIT_FAILURE); }
+        rp->params = p_;
+        rp->hParams = cp[0];
+        rp->dParams = cp[1];
+        rp->eParams = cp[2];
+        rp->currentIteration = iteration;
+        pthread_t w;
+        if (pthread_create(&w, nullptr, &Pipeline::writer_main, job) != 0) {
+            printf("Couldn't start a writer thread\n");
+            exit(EXIT_FAILURE);
+        }
+        pthread_detach(w);
+    }
+
+    // Waits until every dump is written.
+    void drain()
+    {
+        for (Set &s : sets_) join_copies(s);
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return writers_ == 0; });
+    }
+
+    double wait_seconds() const { return wait_s_; }
+
+private:
+    struct Set {
+        float *dev = nullptr, *pinned = nullptr;
+        hipStream_t stream = nullptr;
+        hipEvent_t ready = nullptr;
+        pthread_t copier[3];
+        bool copying = false;
+    };
+    struct Job {
+        Pipeline *self;
+        ResultsParams *rp;
+    };
+
+    static void join_copies(Set &s)
+    {
+        if (!s.copying) return;
+        for (pthread_t &t : s.copier) pthread_join(t, nullptr);
+        s.copying = false;
+    }
+
+    static void *writer_main(void *arg)
+    {
+        Job *job = (Job *)arg;
+        Pipeline *self = job->self;
+        if (self->lines_) write_lines_with_params(job->rp);
+        else writeResultsWithParams(job->rp);
+        delete job;
+        {
+            std::lock_guard<std::mutex> lk(self->m_);
+            self->writers_--;
+        }
+        self->cv_.notify_all();
+        return nullptr;
+    }
+
+    FdtdParams *p_;
+    bool lines_;
+    int writers_max_;
+    size_t n_;
+    std::vector<Set> sets_;
+    int next_ = 0;
+    std::mutex m_;
+    std::condition_variable cv_;
+    int writers_ = 0;
+    double wait_s_ = 0.0;
+};
 
 }  // namespace
 
@@ -121,6 +315,7 @@ int main(int argc, char **argv)
     const size_t bytes = n * sizeof(float);
     hipStream_t stream;
     CHECK(hipStreamCreate(&stream));
+    // --sync: the nine volumes to host on the stepping thread, then the text, then the next step
     std::vector<float *> host(9);
     auto take = [&](const std::vector<const float *> &dev) {
         for (int f = 0; f < 9; f++) {
@@ -130,6 +325,10 @@ int main(int argc, char **argv)
         }
         CHECK(hipStreamSynchronize(stream));
     };
+    // two staging sets: one filled on the device while the other is copied out
+    const bool dumps = params->iterationsCount >= opt.every;
+    Pipeline *pipe = (!opt.sync && dumps) ? new Pipeline(params, opt.lines, 2, opt.writers) : nullptr;
+    const auto t_start = std::chrono::steady_clock::now();
 
     if (opt.legacy) {
         printf("Initializing device field...\n");
@@ -156,11 +355,20 @@ int main(int argc, char **argv)
             CHECK_RC(updateMurBoundary(E[t][0], E[t][1], E[t][2], E[s1][0], E[s1][1], E[s1][2], dev->rpx0, dev->rpy0,
                                        dev->rpz0, dev->rpxEnd, dev->rpyEnd, dev->rpzEnd, stream));
             if ((it + 1) % opt.every == 0) {
-                take({D[t][0], D[t][1], D[t][2], dev->hx, dev->hy, dev->hz, E[t][0], E[t][1], E[t][2]});
-                write_set(params, host.data(), it, opt.lines);
+                const float *src[9] = {D[t][0], D[t][1], D[t][2], dev->hx, dev->hy, dev->hz, E[t][0], E[t][1], E[t][2]};
+                if (pipe) {
+                    float *stage = pipe->begin();
+                    for (int f = 0; f < 9; f++)
+                        CHECK(hipMemcpyAsync(stage + f * n, src[f], bytes, hipMemcpyDeviceToDevice, stream));
+                    pipe->commit(stream, it);
+                } else {
+                    take({src[0], src[1], src[2], src[3], src[4], src[5], src[6], src[7], src[8]});
+                    write_set(params, host.data(), it, opt.lines);
+                }
             }
         }
         CHECK(hipStreamSynchronize(stream));
+        if (pipe) pipe->drain();
         deallocDeviceField(dev);
     } else {
         FdtdExt ext;
@@ -177,17 +385,29 @@ int main(int argc, char **argv)
             if (!opt.quiet) printf("Running iteration %d\n", it + 1);
             CHECK_RC(fdtd_step_fused(ctx, stream));
             if ((it + 1) % opt.every == 0) {
-                for (int f = 0; f < 9; f++) {
-                    host[f] = (float *)malloc(bytes);
-                    if (!host[f]) { printf("Couldn't allocate %zu bytes\n", bytes); exit(EXIT_FAILURE); }
-                    CHECK_RC(fdtd_ctx_get_field(ctx, f, 0, host[f]));
+                if (pipe) {
+                    float *stage = pipe->begin();
+                    CHECK_RC(fdtd_ctx_snapshot(ctx, stage, stream));
+                    pipe->commit(stream, it);
+                } else {
+                    for (int f = 0; f < 9; f++) {
+                        host[f] = (float *)malloc(bytes);
+                        if (!host[f]) { printf("Couldn't allocate %zu bytes\n", bytes); exit(EXIT_FAILURE); }
+                        CHECK_RC(fdtd_ctx_get_field(ctx, f, 0, host[f]));
+                    }
+                    write_set(params, host.data(), it, opt.lines);
                 }
-                write_set(params, host.data(), it, opt.lines);
             }
         }
         CHECK(hipStreamSynchronize(stream));
+        if (pipe) pipe->drain();
         fdtd_ctx_destroy(ctx);
     }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (!opt.quiet || getenv("FDTD_TIMING"))
+        printf("Stepping and output: %.3f s; the stepping thread waited %.3f s for the result writers\n", wall,
+               pipe ? pipe->wait_seconds() : 0.0);
+    delete pipe;
     CHECK(hipStreamDestroy(stream));
     deallocField(field);
     deallocParams(params);

@@ -412,7 +412,10 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         }
         __syncthreads();  // H^{n+1/2} of plane k and its halo staged
 
-        const bool rdx = dx_ij && kz, rdy = dy_ij && kz, rdz = dz_ij && kh;
+        // A source cell's Dz is written even outside the Dz update range (the
+        // upper x/y faces and the top plane), as updateSources does
+        // (fdtd_calculations.cu:187-211).
+        const bool rdx = dx_ij && kz, rdy = dy_ij && kz, rdz = (dz_ij && kh) || (cu.id & kSourceBit);
         const bool reyz = e_i && (k >= 1);
         const float dxu = cu.dxn + cdy * (hz - sH[2][ty][tx + 1]) - cdz * (hy - hym);
         const float dyu = cu.dyn + cdz * (hx - hxm) - cdx * (hz - sH[2][ty + 1][tx]);
@@ -777,13 +780,13 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
             const bool own = h ? own_b : own_a;
             rdx[h] = (i < nx - 1 && j >= 1 && j < ny - 1) && kz;
             rdy[h] = (i >= 1 && i < nx - 1 && j < ny - 1) && kz;
-            rdz[h] = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) && kh;
+            const uint32_t id = (cu.id >> (8 * h)) & 0xffu;
+            rdz[h] = ((i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) && kh) || (id & kSourceBit);
             reyz[h] = (i >= 1) && (k >= 1);
             const float dxn_ = half_of(cu.dxn, h), dyn_ = half_of(cu.dyn, h), dzn_ = half_of(cu.dzn, h);
             dxu[h] = dxn_ + cdy * (hz[h] - lx(sH[2][ty], c)) - cdz * (hy[h] - hym[h]);
             dyu[h] = dyn_ + cdz * (hx[h] - hxm[h]) - cdx * (hz[h] - lx(sH[2][ty + 1], c - 1));
             float dz_ = dzn_ + cdx * (hy[h] - lx(sH[1][ty + 1], c - 1)) - cdy * (hx[h] - lx(sH[0][ty], c));
-            const uint32_t id = (cu.id >> (8 * h)) & 0xffu;
             dz_ = (id & kSourceBit) ? dz_ - jzv : dz_;
             dzu[h] = dz_;
             float dyp = dyu[h], dzp = dz_;
@@ -1255,7 +1258,50 @@ __global__ __launch_bounds__(256) void plane_copy(const PlaneCopy P)
         dst[w] = src[w];
 }
 
+// Unpitches (and for scaled storage, unscales) the nine fields of a context
+// into packed fp32 volumes: lanes run along x, blockIdx.y is the field.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(256) void field_snapshot(const SnapArgs A)
+{
+    const int f = (int)blockIdx.y;
+    const T *src = (const T *)A.field[0];
+    double factor = A.factor[0];
+#pragma unroll
+    for (int q = 1; q < 9; q++)
+        if (f == q) { src = (const T *)A.field[q]; factor = A.factor[q]; }
+    const long long n = (long long)A.nx * A.ny * A.nz;
+    float *out = A.out + (long long)f * n;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+        const long long row = t / A.nx;
+        const int i = (int)(t - row * A.nx);
+        const long long at = row * A.pitch + i;   // row = k*ny + j, and plane = pitch*ny
+        if constexpr (SCALED) {
+            float v;
+            if constexpr (std::is_same<T, float>::value) v = src[at];
+            else v = Elem<T>::cvt(((const unsigned short *)src)[at]);
+            out[t] = (float)((double)v * factor);
+        } else {
+            out[t] = src[at];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_snapshot(const SnapArgs &a, int storage, hipStream_t s)
+{
+    const long long n = (long long)a.nx * a.ny * a.nz;
+    if (n <= 0 || !a.out) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 4096), 9);
+    switch (storage) {
+    case kStoreF32: field_snapshot<float, false><<<grid, 256, 0, s>>>(a); break;
+    case kStoreF32Fast: field_snapshot<float, true><<<grid, 256, 0, s>>>(a); break;
+    case kStoreF16: field_snapshot<__half, true><<<grid, 256, 0, s>>>(a); break;
+    case kStoreBF16: field_snapshot<__hip_bfloat16, true><<<grid, 256, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_plane_copy(const PlaneCopy &p, hipStream_t s)
 {
@@ -1453,6 +1499,25 @@ namespace {
 // The fused step's memory stream without its arithmetic: 15 arrays read and
 // 9 written, element for element, every access nontemporal; a grid-stride
 // loop over n elements of each array.
+// The same mix over row records: the 24 rows of `row` cells that one row of
+// the grid touches are contiguous, so a wave's 24 accesses fall in one
+// record instead of 24 arrays far apart.
+__global__ __launch_bounds__(256) void stream_mix_rows(const StreamMix p)
+{
+    const float *base = p.r[0];
+    float *wbase = const_cast<float *>(base);
+    const long long W = p.row;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < p.n; q += (long long)gridDim.x * 256) {
+        const long long r = q / W, x = q - r * W;
+        const long long rec = r * 24 * W + x;
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 15; a++) acc += __builtin_nontemporal_load(&base[rec + a * W]);
+#pragma unroll
+        for (int b = 15; b < 24; b++) __builtin_nontemporal_store(acc, &wbase[rec + b * W]);
+    }
+}
+
 __global__ __launch_bounds__(256) void stream_mix(const StreamMix p)
 {
     for (long long q = blockIdx.x * 256LL + threadIdx.x; q < p.n; q += (long long)gridDim.x * 256) {
@@ -1468,7 +1533,8 @@ __global__ __launch_bounds__(256) void stream_mix(const StreamMix p)
 
 hipError_t launch_stream_mix(const StreamMix &p, int blocks, hipStream_t s)
 {
-    stream_mix<<<blocks, 256, 0, s>>>(p);
+    if (p.row > 0) stream_mix_rows<<<blocks, 256, 0, s>>>(p);
+    else stream_mix<<<blocks, 256, 0, s>>>(p);
     return hipGetLastError();
 }
 

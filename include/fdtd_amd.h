@@ -92,8 +92,11 @@ int fdtd_ctx_set_material_arrays(fdtd_ctx *ctx, const float *sigma, const float 
 /* The six Mur planes in the reference's layout (fdtd.cu:716-722). */
 int fdtd_ctx_set_rp(fdtd_ctx *ctx, const float *rpx0, const float *rpy0, const float *rpz0,
                     const float *rpxEnd, const float *rpyEnd, const float *rpzEnd);
-/* Soft Jz sources (0-based cells, 1 <= x < nx-1, 1 <= y < ny-1,
- * 0 <= z < nz-1) sharing one waveform jz[jz_len]. */
+/* Soft Jz sources (0-based cells, 1 <= x < nx, 1 <= y < ny, 0 <= z < nz:
+ * the cells updateSources can address, fdtd_calculations.cu:198-211, whose
+ * Hy(x-1) and Hx(y-1) reads need x, y >= 1) sharing one waveform
+ * jz[jz_len].  A source on an upper x/y face or on the top plane sets Dz
+ * there although the D update skips those cells, as the reference does. */
 int fdtd_ctx_set_sources(fdtd_ctx *ctx, int nsrc, const int *xyz, int jz_len, const float *jz);
 /* Point probes (field id, i, j, k); one row of nprobe doubles is recorded on
  * the device after every step, up to `capacity` rows. */
@@ -172,6 +175,16 @@ int fdtd_step_fused(fdtd_ctx *ctx, hipStream_t stream);
 int fdtd_run(fdtd_ctx *ctx, int steps, hipStream_t stream);
 int fdtd_ctx_steps_done(fdtd_ctx *ctx);
 
+/* Enqueues on `stream` a copy of the newest level of all nine fields into
+ * dev_out (device memory of 9*nx*ny*nz floats): field f (FDTD_DX ..
+ * FDTD_EZ) at dev_out + f*nx*ny*nz, cell (i, j, k) at k*ny*nx + j*nx + i
+ * (the reference's OFFSET layout, utils.h:15), in SI units whatever the
+ * storage (the values fdtd_ctx_get_field returns).  Stepping may continue
+ * on `stream` at once: the copy is ordered before the next step.  The
+ * device half of an overlapped result dump (the D2H copy and the text
+ * writers of fdtd.cu:134-155, 1136-1194 then run beside the steps). */
+int fdtd_ctx_snapshot(fdtd_ctx *ctx, float *dev_out, hipStream_t stream);
+
 /* Copies up to max_rows recorded probe rows to host (synchronises the
  * context's streams); returns the number of rows or a negative code. */
 int fdtd_probe_read(fdtd_ctx *ctx, double *host, int max_rows);
@@ -209,6 +222,12 @@ int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
  * 96 bytes per cell.  gbs receives 96 * cells bytes over the mean pass
  * time, ms_per_pass (optional) that time. */
 int fdtd_stream_probe(long long cells, int blocks, int reps, double *gbs, double *ms_per_pass);
+/* The same mix in another layout: row = 0 is fdtd_stream_probe's 24
+ * separate arrays; row > 0 interleaves them by rows of `row` elements (the
+ * 24 rows that cells [q*row, (q+1)*row) touch are contiguous, a 96*row-byte
+ * record), so every wave's accesses fall in one record.  cells must be a
+ * multiple of row. */
+int fdtd_stream_probe_layout(long long cells, int row, int blocks, int reps, double *gbs, double *ms_per_pass);
 
 /* ---- voxel phantoms (fdtd.cu:875-939) ------------------------------------ */
 

@@ -22,12 +22,30 @@ def test_more_than_one_gpu_is_refused():
 
 def test_profiled_traffic_matches_the_launch_shape():
     # the autotuned 256^3 launch and the library default both have counters
-    t = bench.profiled_traffic("plain_256", "f32", "128x2x6")
-    t0 = bench.profiled_traffic("plain_256", "f32", "64x4x4")
-    assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t
+    t, p = bench.profiled_traffic("plain_256", "f32", "128x2x6")
+    t0, _ = bench.profiled_traffic("plain_256", "f32", "64x4x4")
+    assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t and p.endswith(".md")
+    # the headline workload at the default launch shape has its own counter passes
+    tl, pl = bench.profiled_traffic("large_phantom", "f32", "64x4x4")
+    assert tl and 1.0 < tl / (97 * 320 * 320 * 880) < 1.8
     # a shape without counters, or 16-bit storage, gives none
-    assert bench.profiled_traffic("plain_256", "f32", "64x2x3") is None
-    assert bench.profiled_traffic("plain_256", "f16", "128x2x6") is None
+    assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
+    assert bench.profiled_traffic("plain_256", "f16", "128x2x6") == (None, None)
+
+
+def test_headline_defaults():
+    """With no flags the bench measures the 320x320x880 phantom at the
+    library's default launch shape after an untimed advance, and times at
+    least 64 fused launches for the roofline."""
+    import argparse
+    old = sys.argv
+    try:
+        sys.argv = ["bench.py"]
+        a = bench.parse()
+    finally:
+        sys.argv = old
+    assert isinstance(a, argparse.Namespace)
+    assert a.workload == "large_phantom" and not a.autotune and a.advance >= 200 and a.timing_launches >= 64
 
 
 def test_cpu_baseline_fields():

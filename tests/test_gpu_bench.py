@@ -22,14 +22,15 @@ def _bench(*args):
 
 
 def test_bench_line_contract():
-    d = _bench("--workload", "plain_small", "--steps", "32", "--warmup", "2", "--no-cpu-baseline",
-               "--time-every", "4")
+    d = _bench("--workload", "plain_small", "--steps", "32", "--warmup", "2", "--advance", "40",
+               "--no-cpu-baseline", "--timing-launches", "8")
     for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
               "vs_baseline", "dtype", "data", "config", "roofline"):
         assert k in d, k
     assert d["n_gpus"] == 1 and d["steps"] == 32 and d["value"] > 0 and d["probes_finite_nonzero"]
     r = d["roofline"]
-    assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["launches_timed"] == 8
+    assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["launches_timed"] == 64   # at least 64
+    assert d["config"]["timed_steps"] == [42, 74] and r["timed_launch_steps"] == [74, 138]
     assert 0 < r["frac"] < 1 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
     assert r["traffic"] is None      # no counter profile for this workload
     c = r["stream_ceiling"]
@@ -39,7 +40,7 @@ def test_bench_line_contract():
 
 def test_bench_16bit_line_has_its_ceiling():
     d = _bench("--workload", "plain_small", "--storage", "f16", "--steps", "16", "--warmup", "2",
-               "--no-cpu-baseline", "--no-autotune")
+               "--no-cpu-baseline")
     r = d["roofline"]
     assert r["bytes_per_cell"] == 49 and d["dtype"].startswith("f16")
     c = r["stream_ceiling"]

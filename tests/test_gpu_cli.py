@@ -121,3 +121,22 @@ def test_cli_line_files_match_oracle(tmp_path):
                 want = " %3d %3d %3d" % (i + 1, j + 1, k + 1) + "".join(" % 9.3E" % fields[f][k, j, i]
                                                                        for f in range(9))
                 assert lines[q] == want, (axis, it, q)
+
+
+@pytest.mark.parametrize("legacy", [False, True])
+def test_cli_overlapped_output_equals_sync(tmp_path, legacy):
+    """Dumps written by the overlapped pipeline (device staging, pinned
+    D2H on a side stream, copy and writer threads) equal, byte for byte,
+    those written on the stepping thread after each step (--sync); with one
+    writer in flight and with seven."""
+    print(f"cli: overlapped vs --sync, every step, legacy={legacy}", flush=True)
+    params, text = _case(tmp_path, t_max=9)
+    extra = ("--legacy",) if legacy else ()
+    sync = _run(params, text, tmp_path / "out_sync", "--every", "1", "--sync", *extra)
+    one = _run(params, text, tmp_path / "out_one", "--every", "1", "--writers", "1", *extra)
+    seven = _run(params, text, tmp_path / "out_seven", "--every", "1", *extra)
+    assert len(sync) == 9 * 9 * SHAPE[2]
+    assert one == sync and seven == sync
+    lines = _run(params, text, tmp_path / "out_lines", "--every", "2", "--lines", *extra)
+    lines_sync = _run(params, text, tmp_path / "out_lines_sync", "--every", "2", "--lines", "--sync", *extra)
+    assert lines == lines_sync and len(lines) == 4 * 3

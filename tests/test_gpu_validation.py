@@ -30,8 +30,8 @@ def test_rejected_calls_leave_the_context_unchanged():
     a = _ctx(ids, table, src, jz)
     a.set_probes(np.asarray([[8, 10, 8, 6]], np.int32), 8)
     bad = [
-        (lambda: a.set_sources(np.asarray([[0, 8, 6]], np.int32), jz), -2),        # x = 0: no Dz update there
-        (lambda: a.set_sources(np.asarray([[10, 8, nz - 1]], np.int32), jz), -2),   # z = nz-1
+        (lambda: a.set_sources(np.asarray([[0, 8, 6]], np.int32), jz), -2),        # x = 0: updateSources would read Hy(x-1) outside the row
+        (lambda: a.set_sources(np.asarray([[10, 8, nz]], np.int32), jz), -2),       # z = nz
         (lambda: a.set_probes(np.asarray([[9, 1, 1, 1]], np.int32), 8), -2),        # field 9
         (lambda: a.set_probes(np.asarray([[8, nx, 1, 1]], np.int32), 8), -2),       # i = nx
         (lambda: a.set_materials(np.full(ids.shape, len(table), np.uint8), table), -2),  # id >= nmat

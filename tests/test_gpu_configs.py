@@ -139,9 +139,13 @@ def test_served_stream_prefix_alone_and_mixed():
     print(f"[configs] fp32 requests: worst probe rel-L2 against FP64 {worst:.1e}", flush=True)
 
 
-# per-field rel-L2 bounds against FP64 after 60 steps (Dx Dy Dz Hx Hy Hz Ex Ey Ez) and on the probe
-# beside the dipole, for fp16 and bf16 storage
-BOUNDS_16 = {"f16": (0.1, 0.05), "bf16": (1.0, 0.5)}
+# rel-L2 bounds against FP64 after 60 steps, per field (Dx Dy Dz Hx Hy Hz Ex Ey Ez), and on the probe
+# beside the dipole, for fp16 and bf16 storage.  Measured on MI355X: fp16 Dx 9.1e-4 Dy 1.2e-3 Dz 1.8e-3
+# Hx 5.6e-3 Hy 4.2e-3 Hz 6.8e-2 Ex 6.8e-4 Ey 8.5e-4 Ez 7.7e-4, probe 3.0e-3; bf16 Dx 7.4e-3 Dy 9.1e-3
+# Dz 1.2e-2 Hx 4.3e-2 Hy 3.9e-2 Hz 0.64 Ex 6.3e-3 Ey 7.3e-3 Ez 6.8e-3, probe 1.5e-2.  Hz, the smallest
+# component beside a z-directed dipole, carries the largest relative error.
+BOUNDS_16 = {"f16": ((3e-3, 3e-3, 3e-3, 1e-2, 1e-2, 0.1, 2e-3, 2e-3, 2e-3), 5e-3),
+             "bf16": ((2e-2, 2e-2, 2e-2, 0.1, 0.1, 1.0, 2e-2, 2e-2, 2e-2), 3e-2)}
 
 
 def test_large_phantom_16bit_storage_against_fp64():
@@ -162,4 +166,6 @@ def test_large_phantom_16bit_storage_against_fp64():
         print(f"[configs] {name}: fields " + " ".join(f"{F.FIELDS[f]} {e:.1e}" for f, e in enumerate(errs))
               + f"; probe beside the dipole {near:.1e}", flush=True)
         fb, pb = BOUNDS_16[name]
-        assert max(errs) <= fb and near <= pb, name
+        for f in range(9):
+            assert errs[f] <= fb[f], f"{name} {F.FIELDS[f]}: rel-L2 {errs[f]:.2e} against FP64"
+        assert near <= pb, f"{name}: probe beside the dipole rel-L2 {near:.2e}"

@@ -52,7 +52,7 @@ EXPORTS = {
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
-                   "fdtd_dropin_solve", "fdtd_stream_probe", "fdtd_stream_probe_layout", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
+                   "fdtd_dropin_solve", "fdtd_dropin_solve_noalloc", "fdtd_stream_probe", "fdtd_stream_probe_layout", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
                    "fdtd_serve_result", "fdtd_serve_release", "fdtd_serve_drain", "fdtd_serve_stats"],
 }
@@ -193,6 +193,7 @@ def lib():
         L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_batch_info.argtypes = [vp, C.POINTER(FdtdBatchInfo)]
         L.fdtd_dropin_solve.argtypes = [PP, PE, PI, vp, vp]
+        L.fdtd_dropin_solve_noalloc.argtypes = [PP, PE, PI, vp, vp]
         L.fdtd_stream_probe.argtypes = [C.c_longlong, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fdtd_stream_probe_layout.argtypes = [C.c_longlong, i, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fdtd_load_tissue_table.argtypes = [C.c_char_p, vp]
@@ -521,6 +522,15 @@ def dropin_solve(inst, stream=None):
     out = np.zeros((inst.steps, max(inst.nprobe, 1)), np.float64)
     check(lib().fdtd_dropin_solve(C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c), _ptr(out), stream),
           "fdtd_dropin_solve")
+    return out[:, :inst.nprobe]
+
+
+def dropin_solve_noalloc(inst, stream=None):
+    """As dropin_solve, with the grid's state in the library's static device
+    array instead of allocations (fdtd_dropin_solve_noalloc)."""
+    out = np.zeros((inst.steps, max(inst.nprobe, 1)), np.float64)
+    check(lib().fdtd_dropin_solve_noalloc(C.byref(inst.params), C.byref(inst.ext), C.byref(inst.c), _ptr(out),
+                                          stream), "fdtd_dropin_solve_noalloc")
     return out[:, :inst.nprobe]
 
 

@@ -330,6 +330,19 @@ int fdtd_batch_info(fdtd_batch *b, FdtdBatchInfo *info);
 int fdtd_dropin_solve(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst, double *probes_host,
                       hipStream_t stream);
 
+/* fdtd_dropin_solve without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops and harnesses that forbid hipMalloc).  The
+ * grid's state (21 field and 4 material arrays, rp planes, ids, table,
+ * probe rows) lives in a static device array of 64 MiB per GPU, held until
+ * the process ends: a grid whose state exceeds it (about 640 K cells, e.g.
+ * 86^3) is refused with FDTD_ERANGE.  Same inputs, outputs and results as
+ * fdtd_dropin_solve, bit for bit; calls no hipMalloc/hipFree.  All calls on
+ * one GPU share that array (and the process-global symbols), so calls that
+ * overlap on one GPU race; each call clears the state it uses first. */
+int fdtd_dropin_solve_noalloc(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *inst,
+                              double *probes_host, hipStream_t stream);
+
 /* ---- serving ------------------------------------------------------------
  *
  * A server owns a batch and a worker thread.  Requests (a grid, its inputs

@@ -101,6 +101,9 @@ def parse():
                     help="pick the launch shape by timing each candidate first (default: the library's "
                          "default shape, 64x4 tiles of 4 planes, so the command's profile is reproducible)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--load-zero-terms", action="store_true",
+                    help="fp32: load E^{n-1} and D^{n-1} on every row, also where the Debye update multiplies "
+                         "them by zero (tau = 0); the default skips those loads, bit for bit the same results")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape TXxTYxKC (threads per tile row, tile rows, z planes per block), "
                          "e.g. 128x2x6; no autotuning")
@@ -149,6 +152,24 @@ def profiled_traffic(workload, storage, launch):
         if rd and wr:
             return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024, path
     return None, None
+
+
+def zero_term_fraction(w, shape):
+    """Fraction of the fused kernel's (64-cell row segment, plane) units that
+    skip loading E^{n-1} and D^{n-1}: every cell of the segment has tau = 0
+    (exactly +0), and the kernel has decided for the plane (a z-chunk's first
+    plane always loads, and the bottom chunk decides from its third plane)."""
+    nx, ny, nz = w.shape
+    kc = shape["kchunk"]
+    zero_mat = np.asarray([np.float32(t[3]).view(np.uint32) == 0 for t in w.table])
+    zc = zero_mat[w.ids]                                   # (nz, ny, nx)
+    segs = (nx + 63) // 64
+    pad = np.ones((nz, ny, segs * 64), bool)
+    pad[:, :, :nx] = zc
+    seg_ok = pad.reshape(nz, ny, segs, 64).all(axis=3)     # (nz, ny, segs)
+    k = np.arange(nz)
+    decided = ((k % kc) != 0) & ~((k < kc) & (k == 1))
+    return float((seg_ok & decided[:, None, None]).sum() / seg_ok.size)
 
 
 def host_cpu():
@@ -233,7 +254,7 @@ def main():
     if a.shape:
         tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
         fixed = dict(tile_x=tx, tile_y=ty, kchunk=kc, lane_cells=1)
-    ctx = F.Context(w.params, storage=storage, **fixed)
+    ctx = F.Context(w.params, storage=storage, load_zero_terms=int(a.load_zero_terms), **fixed)
     ctx.set_materials(w.ids, w.table)
     ctx.set_sources(w.sources, w.jz)
     ctx.set_probes(w.probes, total)
@@ -303,6 +324,14 @@ def main():
                      "bytes_per_cell": bpc},
         "probes_finite_nonzero": ok,
     }
+    if a.storage == "f32":
+        # rows of the fused kernel that skip E^{n-1}, D^{n-1} (tau = 0 on all their cells): the bytes the
+        # kernel needs per cell beside the 97 B/cell model
+        zf = zero_term_fraction(w, shape)
+        out["roofline"]["zero_term_rows"] = round(zf, 4) if not a.load_zero_terms else 0.0
+        need = bpc - 24.0 * (zf if not a.load_zero_terms else 0.0)
+        out["roofline"]["bytes_per_cell_needed"] = round(need, 2)
+        out["roofline"]["achieved_needed_bytes"] = round(need * cells / kavg_s / 1e9, 1)
     ctx.close()
     # the same box's streaming ceiling for this access mix (fdtd_stream_probe), a yardstick beside the
     # peak.  16-bit fields move as 32-bit words of two cells, so their probe streams half as many words.
@@ -521,8 +550,9 @@ def bench_slabs(a):
     print(json.dumps(out), flush=True)
 
 
-def _instance(F, w, storage="f32"):
-    ext = F.make_ext(storage={"f32": F.STORE_F32, "f16": F.STORE_F16, "bf16": F.STORE_BF16}[storage])
+def _instance(F, w, storage="f32", load_zero_terms=0):
+    ext = F.make_ext(storage={"f32": F.STORE_F32, "f16": F.STORE_F16, "bf16": F.STORE_BF16}[storage],
+                     load_zero_terms=load_zero_terms)
     return F.Instance(w.params, ext, w.ids, w.table, w.sources, w.jz, w.steps, w.probes)
 
 
@@ -569,7 +599,7 @@ def bench_batched(a):
     from fdtd_amd import workloads as W
 
     ws = W.batched_set()
-    insts = [_instance(F, w, a.storage) for w in ws]
+    insts = [_instance(F, w, a.storage, int(a.load_zero_terms)) for w in ws]
     work = sum(w.cells * w.steps for w in ws)
     cells = sum(w.cells for w in ws)
     arena = int(sum(x.nbytes() for x in insts) * 1.02) + (64 << 20)
@@ -639,7 +669,7 @@ def bench_served(a):
     from fdtd_amd import workloads as W
 
     stream = W.served_stream(a.requests, f16_share=a.f16_share)
-    reqs = [(t, _instance(F, w, st), st) for t, w, st in stream]
+    reqs = [(t, _instance(F, w, st, int(a.load_zero_terms)), st) for t, w, st in stream]
     work = sum(x.params.nx * x.params.ny * x.params.nz * x.steps for _, x, _ in reqs)
     srv = F.Server(int(a.budget_gb * (1 << 30)), max_slots=a.slots, segment_steps=a.segment)
     ids = [None] * len(reqs)

@@ -95,7 +95,7 @@ class FdtdExt(C.Structure):
     _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
                 ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int), ("storage", C.c_int),
                 ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float), ("lane_cells", C.c_int),
-                ("tile_x", C.c_int)]
+                ("tile_x", C.c_int), ("load_zero_terms", C.c_int)]
 
 
 class FdtdInstance(C.Structure):
@@ -256,7 +256,7 @@ class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
     def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0,
-                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0):
+                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0, load_zero_terms=0):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))
@@ -266,6 +266,7 @@ class Context:
         ext.scale_e, ext.scale_d, ext.scale_h = scales
         ext.lane_cells = lane_cells
         ext.tile_x = tile_x
+        ext.load_zero_terms = load_zero_terms
         h = C.c_void_p()
         self._params = params
         check(L.fdtd_ctx_create(C.byref(params), C.byref(ext), C.byref(h)), "fdtd_ctx_create")
@@ -429,13 +430,14 @@ class Context:
 
 
 def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, scales=(0.0, 0.0, 0.0),
-             tile_y=0, kchunk=0, xcd_remap=0):
+             tile_y=0, kchunk=0, xcd_remap=0, load_zero_terms=0):
     ext = FdtdExt()
     lib().fdtd_ext_defaults(C.byref(ext))
     ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
     ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
     ext.storage = storage
     ext.scale_e, ext.scale_d, ext.scale_h = scales
+    ext.load_zero_terms = load_zero_terms
     return ext
 
 

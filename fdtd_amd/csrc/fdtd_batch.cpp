@@ -520,6 +520,8 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     d.dt = p->dt; d.hx = p->dx; d.hy = p->dy; d.hz = p->dz; d.mu0 = p->mu0; d.eps0 = p->eps0;
     d.murx0 = (const float4 *)(base + L.murx);
     d.murx1 = d.murx0 + (size_t)ny * nz;
+    // zero-term rows: exact fp32 form; batch grids start from zero fields
+    d.zskip = (e.storage == FDTD_STORE_F32 && e.load_zero_terms == 0) ? 1 : 0;
     I.start = b->step;
     I.used = true;
     b->dirty = true;

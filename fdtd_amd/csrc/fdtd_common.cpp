@@ -253,6 +253,8 @@ int fdtd_check_ext(const FdtdExt &e)
     if (!(e.scale_e >= 0.0f && e.scale_d >= 0.0f && e.scale_h >= 0.0f)) return fdtd_fail(FDTD_EINVAL, "negative scale");
     if (e.lane_cells < 0 || e.lane_cells > 2) return fdtd_fail(FDTD_EINVAL, "lane_cells %d", e.lane_cells);
     if (e.tile_x != 0 && e.tile_x != 64 && e.tile_x != 128) return fdtd_fail(FDTD_EINVAL, "tile_x %d", e.tile_x);
+    if (e.load_zero_terms != 0 && e.load_zero_terms != 1)
+        return fdtd_fail(FDTD_EINVAL, "load_zero_terms %d", e.load_zero_terms);
     return 0;
 }
 

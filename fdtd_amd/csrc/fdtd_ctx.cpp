@@ -174,6 +174,7 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->scale_e = ext->scale_d = ext->scale_h = 0.0f;
     ext->lane_cells = 0;
     ext->tile_x = 0;
+    ext->load_zero_terms = 0;
     return 0;
 }
 
@@ -474,6 +475,9 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     a.txw = c->txw;
     a.murx0 = c->murx;
     a.murx1 = c->murx + (size_t)c->ny * c->nz;
+    // zero-term rows rely on E^{n-1}, D^{n-1} being finite wherever E^n, D^n are, which stepping from
+    // zero (or from any state the steps made) keeps; a state set from outside may not
+    a.zskip = (c->storage == FDTD_STORE_F32 && c->ext.load_zero_terms == 0 && !c->fields_set) ? 1 : 0;
     const bool timed = c->tused < c->tcap && c->tseen++ % c->tevery == 0;
     hipError_t e = timed ? launch_fused_step(a, c->ty, c->storage, stream, c->tev[2 * c->tused], c->tev[2 * c->tused + 1])
                          : launch_fused_step(a, c->ty, c->storage, stream);

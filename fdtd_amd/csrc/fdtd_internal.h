@@ -64,6 +64,10 @@ struct StepArgs {
     const float4 *murx0, *murx1;    // per (k, j) coefficients of the x-low / x-high faces (mur_x_coefs)
     int cpl;                        // cells per lane: 1, or 2 for 16-bit storage (128-cell tiles)
     int txw;                        // threads per tile row with one cell per lane: 64 or 128
+    // Exact fp32 form: a wave row whose cells all have materials whose Debye
+    // update multiplies E^{n-1} and D^{n-1} by +0 (a = +0 and g3 = +0, i.e.
+    // tau = +0) does not load those six operands (see fused_tile).
+    int zskip;
 };
 
 // Coefficients {q, A, B, C} of the first-order Mur update of one x-face
@@ -267,6 +271,7 @@ struct BatchInst {
     int mur_x;                     // see StepArgs
     const float4 *murx0, *murx1;   // see StepArgs
     int cpl;                       // cells per lane (kernel class)
+    int zskip;                     // see StepArgs
     int nprobe, probe_rows;        // probe_out holds probe_rows rows
     float dt, hx, hy, hz, mu0, eps0;
     int mur_seg[13];

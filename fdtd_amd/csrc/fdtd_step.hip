@@ -197,10 +197,11 @@ __device__ __forceinline__ void stage_lut(const MatCoef *lut, int nmat, int tid,
 // first LDS read of it follows a barrier inside.  The tile's own LDS writes
 // start at once, so a caller that runs several tiles per block puts a
 // barrier between them.
-template <int TY, typename T, bool EXACT, int TXW = 64>
+template <int TY, typename T, bool EXACT, int TXW = 64, bool ZS = false>
 __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, int bz)
 {
     static_assert(TXW == 64 || TXW == 128, "tile width");
+    static_assert(!ZS || EXACT, "zero-term rows belong to the exact form");
     static_assert(TY >= 2 && TY <= 8, "tile rows");
     static_assert(!EXACT || std::is_same<T, float>::value, "the exact form stores fp32");
     // EXACT reproduces the reference's arithmetic (double Debye sums); the
@@ -281,22 +282,46 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         if (w_col) { sE[buf][0][cr][cl] = pc[0]; sE[buf][1][cr][cl] = pc[1]; sE[buf][2][cr][cl] = pc[2]; }
     };
 
-    // Own-cell operands of one plane.
+    // Own-cell operands of one plane.  zs: E^{n-1} and D^{n-1} were not
+    // loaded (zero-term row, below) and read as +0.
     struct Own {
         float hxo, hyo, hzo, dxn, dyn, dzn, dxm, dym, dzm, exm, eym, ezm, hh0, hh1;
         uint32_t id;
+        bool zs;
     };
-    auto fetch_own = [&](int k, Own &o) {
+    auto fetch_id = [&](int k) -> uint32_t { return __builtin_nontemporal_load(ids + ((uint32_t)k * szb + c_own) / ES); };
+    // load_id: the plane's ids come with its operands (else the caller loaded them ahead)
+    auto fetch_own = [&](int k, Own &o, bool zs, bool load_id) {
         const uint32_t ob = (uint32_t)k * szb + c_own;
         o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
         o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
-        o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
-        o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
-        o.id = __builtin_nontemporal_load(ids + ob / ES);
+        if (ZS && zs) {
+            o.dxm = o.dym = o.dzm = o.exm = o.eym = o.ezm = 0.0f;
+        } else {
+            o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
+            o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+        }
+        if (load_id) o.id = __builtin_nontemporal_load(ids + ob / ES);
+        o.zs = ZS && zs;
         // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TXW, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;
         if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
         if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
+    };
+    // Zero-term rows (ZS, exact form): a material with tau = +0 has a = +0 and
+    // g3 = +0, so its Debye update multiplies E^{n-1} and D^{n-1} by +0.  When
+    // every own cell of the wave's row has such a material, the row's six
+    // E^{n-1}/D^{n-1} loads are skipped (wave-uniform).  The ids move two
+    // planes ahead: the id load of plane k+2 is issued first in iteration k
+    // and decided at its end, so plane k+2's loads go out at once in
+    // iteration k+1.  The first z-chunk decides from its third plane on (no
+    // barrier guards the coefficient table earlier), the others from their
+    // second.
+    const bool zskip = ZS && A.zskip;
+    auto zero_row = [&](uint32_t id) -> bool {
+        const int m = id & kMaterialMask;
+        const bool z = __float_as_uint(lA[m]) == 0u && __double_as_longlong(lG3[m]) == 0ll;   // both +0
+        return __all(!own || z);
     };
 
     // Mur x faces: the tile's coefficients (planes k0..k1-1, rows j0..j0+TY-1,
@@ -323,12 +348,18 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
 
     float hxm = 0.0f, hym = 0.0f;  // own-column Hx, Hy of plane k-1 (time level n+1/2)
     int cur = 0;
+    uint32_t idq = 0;     // ZS: ids of the plane after next, loaded ahead
+    bool zsn = false;     // ZS: the next plane skips its E^{n-1}, D^{n-1} loads
+    if constexpr (ZS) {
+        if (k0 + 1 < k1) idq = fetch_id(k0 + 1);
+    }
     if (k0 > 0) {
         fetch_e(k0 - 1);
         stage_e(0);
         fetch_e(k0);
         stage_e(1);
         __syncthreads();
+        if constexpr (ZS) zsn = zskip && k0 + 1 < k1 && zero_row(idq);   // table staged, idq in
         const int k = k0 - 1;
         const uint32_t ob = (uint32_t)k * szb + c_own;
         const float hx0 = X::ld(hxo, ob), hy0 = X::ld(hyo, ob);
@@ -345,7 +376,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         stage_e(0);
     }
     Own cu, nx_;
-    fetch_own(k0, cu);
+    fetch_own(k0, cu, false, true);
     fetch_e(k0 + 1);
 
     // Range masks that do not depend on k.
@@ -374,7 +405,13 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         stage_e(nxt);                 // E^n of plane k+1
         const bool more = (k + 1 < k1);
         if (more) {
-            fetch_own(k + 1, nx_);
+            if constexpr (ZS) {
+                nx_.id = idq;                                 // plane k+1
+                if (k + 2 < k1) idq = fetch_id(k + 2);
+                fetch_own(k + 1, nx_, zsn, false);
+            } else {
+                fetch_own(k + 1, nx_, false, true);
+            }
             fetch_e(k + 2);
         }
         __syncthreads();  // planes k (cur) and k+1 (nxt) staged
@@ -430,6 +467,21 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         if constexpr (exact) {
             const float ca = lA[m], cc = lC[m], cd = lD[m];
             const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
+            if (ZS && cu.zs) {
+                // The skipped terms are a*E^{n-1} and g3*D^{n-1} with a = g3 = +0, so the
+                // result is the reference's unless d*E^n is -0 (a zero sum then takes the
+                // sign of the dropped term, which may be -0) or E^n or D^n is not finite
+                // (E^{n-1} or D^{n-1} can only be infinite or NaN where they are, and
+                // 0*inf is NaN).  A row with such a cell loads the real operands after all.
+                auto neg0 = [](float v) { return __float_as_uint(v) == 0x80000000u; };
+                auto fin = [](float v) { return __builtin_isfinite(v); };
+                const bool risky = own && (neg0(cd * exk) || neg0(cd * eyk) || neg0(cd * ezk) || !fin(exk) ||
+                                           !fin(eyk) || !fin(ezk) || !fin(cu.dxn) || !fin(cu.dyn) || !fin(cu.dzn));
+                if (__any(risky)) {
+                    cu.exm = X::ld_nt(ex1, ob); cu.eym = X::ld_nt(ey1, ob); cu.ezm = X::ld_nt(ez1, ob);
+                    cu.dxm = X::ld_nt(dx1, ob); cu.dym = X::ld_nt(dy1, ob); cu.dzm = X::ld_nt(dz1, ob);
+                }
+            }
             exn = e_update(cc, cd, ca, g1, g2, g3, exk, cu.exm, dxu, cu.dxn, cu.dxm);
             eyn = e_update(cc, cd, ca, g1, g2, g3, eyk, cu.eym, dyp, cu.dyn, cu.dym);
             ezn = e_update(cc, cd, ca, g1, g2, g3, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
@@ -488,6 +540,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         hym = hy;
         cur = nxt;
         if (more) cu = nx_;
+        if constexpr (ZS) zsn = zskip && k + 2 < k1 && zero_row(idq);   // for plane k+2
     }
 }
 
@@ -848,7 +901,7 @@ __device__ __forceinline__ void fused_tile_pair(const StepArgs &A, int bx, int b
 
 // One grid, one step: block b runs tile xcd_logical(b), at five waves per
 // SIMD (96 VGPRs).
-template <int TY, typename T, bool EXACT, int TXW>
+template <int TY, typename T, bool EXACT, int TXW, bool ZS = false>
 __global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) void fused_step(const StepArgs A)
 {
     const int tid = threadIdx.y * TXW + threadIdx.x;
@@ -858,7 +911,7 @@ __global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) vo
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);
     stage_lut<EXACT>(A.lut, A.nmat, tid, TXW * TY);
-    fused_tile<TY, T, EXACT, TXW>(A, bx, by, bz);
+    fused_tile<TY, T, EXACT, TXW, ZS>(A, bx, by, bz);
 }
 
 // Batch tables are read through the constant address space: they do not
@@ -911,6 +964,7 @@ __device__ __forceinline__ void batch_step_args(Const<BatchInst> &d, int n, Step
     A.mur_x = d.mur_x;
     A.murx0 = d.murx0;
     A.murx1 = d.murx1;
+    A.zskip = d.zskip;
 }
 
 // Persistent fused kernel over every live instance of one storage class:
@@ -950,7 +1004,7 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4
             staged = id;
         }
         if constexpr (PAIR) fused_tile_pair<TY, T>(A, bx, by, bz);
-        else fused_tile<TY, T, EXACT>(A, bx, by, bz);
+        else fused_tile<TY, T, EXACT, 64, EXACT>(A, bx, by, bz);   // zero-term rows per grid (A.zskip)
     }
 }
 
@@ -1377,6 +1431,24 @@ hipError_t launch_fused_t(const StepArgs &a, int ty, hipStream_t s, hipEvent_t t
             case 2: return go((const void *)fused_step_pair<2, T>, TX, 2);
             case 4: return go((const void *)fused_step_pair<4, T>, TX, 4);
             case 8: return go((const void *)fused_step_pair<8, T>, TX, 8);
+            default: return hipErrorInvalidValue;
+            }
+        }
+    }
+    if constexpr (EXACT) {
+        if (a.zskip) {   // zero-term rows
+            if (a.txw == 128) {
+                switch (ty) {
+                case 2: return go((const void *)fused_step<2, T, EXACT, 128, true>, 128, 2);
+                case 4: return go((const void *)fused_step<4, T, EXACT, 128, true>, 128, 4);
+                case 8: return go((const void *)fused_step<8, T, EXACT, 128, true>, 128, 8);
+                default: return hipErrorInvalidValue;
+                }
+            }
+            switch (ty) {
+            case 2: return go((const void *)fused_step<2, T, EXACT, 64, true>, TX, 2);
+            case 4: return go((const void *)fused_step<4, T, EXACT, 64, true>, TX, 4);
+            case 8: return go((const void *)fused_step<8, T, EXACT, 64, true>, TX, 8);
             default: return hipErrorInvalidValue;
             }
         }

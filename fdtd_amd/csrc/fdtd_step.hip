@@ -1109,7 +1109,7 @@ __device__ __forceinline__ float probe_value(const MurArgs &M, Seg seg, const vo
 #pragma unroll
         for (int q = 0; q < 12; q++)
             if (q == s) active = seg[q + 1] > seg[q];
-        if (active) return mur_value<T>(M, s, i, j, k);
+        if (active) return Elem<T>::rnd(mur_value<T>(M, s, i, j, k));   // as mur_cell stores it
     }
     const long long o = (long long)k * M.plane + (long long)j * M.pitch + i;
     return Elem<T>::ld((const T *)field, (uint32_t)(o * (long long)sizeof(T)));

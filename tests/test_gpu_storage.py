@@ -12,7 +12,8 @@ import pytest
 import fdtd_amd as F
 from fdtd_amd import workloads as W
 from oracle import oracle as O
-from tests.helpers import rel_l2
+from tests.helpers import random_materials, random_rp, random_sources, random_state, rel_l2
+from tests.test_gpu_parity import _face_probes, _params
 
 pytestmark = pytest.mark.gpu
 
@@ -194,3 +195,41 @@ def test_16bit_slices_and_probes_read_the_stored_fields(storage):
         want = float(fields[f][k, j, i])
         assert abs(last[p] - want) <= 1e-6 * abs(want) + 1e-30, (p, last[p], want)
     assert np.abs(last).max() > 0
+
+
+@pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16])
+@pytest.mark.parametrize("shape,kchunk,lane_cells", [((40, 21, 19), 4, 1),    # x faces in the fused kernel
+                                                     ((65, 12, 11), 4, 1),    # x-high faces in the Mur kernel
+                                                     ((40, 21, 19), 20, 1),   # every face in the Mur kernel
+                                                     ((129, 12, 11), 4, 2),   # pairs: x-high faces in the Mur kernel
+                                                     ((40, 21, 19), 20, 2)])
+def test_16bit_face_probes_read_the_stored_fields(storage, shape, kchunk, lane_cells):
+    """A probe on a face cell that the Mur launch updates evaluates the face
+    itself in that launch; under 16-bit storage it must report the value as
+    stored (rounded to 16 bits), like every other probe."""
+    print(f"\n16-bit face probes {shape} kc={kchunk} cells/lane={lane_cells} storage={storage}", flush=True)
+    rng = np.random.default_rng(13)
+    ids, table = random_materials(rng, shape)
+    probes = _face_probes(shape)
+    ctx = F.Context(_params(shape), tile_y=4, kchunk=kchunk, storage=storage, lane_cells=lane_cells,
+                    scales=(1.0, 1.0, 1.0))
+    ctx.set_materials(ids, table)
+    ctx.set_rp(random_rp(rng, shape))
+    ctx.set_sources(random_sources(rng, shape, 2), rng.uniform(-1, 1, 10).astype(np.float32))
+    for (f, lv), a in random_state(rng, shape).items():
+        ctx.set(f, lv, a)
+    steps = 6
+    ctx.set_probes(probes, steps)
+    for n in range(steps):
+        ctx.step()
+        row = ctx.probes()[n]
+        if n == 0:
+            assert np.isfinite(row).all()
+        # random state in random media overflows 16 bits within a few steps: infinities and NaN
+        # must match too
+        for p, (f, i, j, k) in enumerate(probes):
+            want = float(ctx.get(f)[k, j, i])
+            same = row[p] == want or (np.isnan(row[p]) and np.isnan(want)) or \
+                abs(row[p] - want) <= 1e-6 * abs(want) + 1e-37
+            assert same, f"step {n + 1} probe {p} {O.FIELDS[f]} ({i},{j},{k}): {row[p]} vs {want}"
+    ctx.close()

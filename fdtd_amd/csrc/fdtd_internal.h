@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <algorithm>
+#include <initializer_list>
 #include <stdint.h>
 
 #include "fdtd_amd.h"
@@ -234,17 +236,38 @@ inline int auto_lane_cells(int nx, size_t esz, int requested)
     return e2 >= 0.85 * e1 ? 2 : 1;
 }
 
-// The same choice for a grid launched alone: a launch with fewer pair tiles
-// than the GPU holds at once leaves CUs idle (measured: 96^3 and
-// 200x100x100 run 9-13% slower with pairs, 256^3 18% faster), so pairs need
-// 128-wide rows that waste little and at least 4096 tiles.
+// Launch shape of a 16-bit grid launched alone (tile rows ty given), from a
+// sweep of 8 grid sizes (64^3 to 320x320x880) x {1, 2} cells per lane x
+// z-chunks {4, 8, 16} x rows {4, 8}, fp16 and bf16, on MI355X:
+//  * a launch needs about 1024 blocks to keep the GPU busy; beyond that,
+//    deeper z-chunks pay (fewer prologues): 320x320x880 runs 1101 us at 16
+//    planes against 1251 at 4, 256^3 pairs 203 against 221;
+//  * pairs (128-cell tiles, one 32-bit access per stream) beat single cells
+//    where 128-cell rows are nearly full (256^3: 203 against 267 us; 128^3:
+//    35 against 45), and lose where they are not (320 wide: 1115 against
+//    1101; 200 wide: 53 against 48), also where 64-cell tiles would waste
+//    as many lanes.
+// So: pairs where 128-cell rows are at least 90% full and a pair launch
+// reaches 1024 blocks; then the deepest of 16, 8, 4 planes that keeps 1024
+// blocks (4 below that).  Every size of the sweep gets its best or
+// near-best shape (within 2%).  `kchunk` or `requested` > 0 fix that part.
+constexpr long long kAloneMinBlocks = 1024;
+inline long long alone_blocks(int nx, int ny, int nz, int width, int ty, int kchunk)
+{
+    return (long long)((nx + width - 1) / width) * ((ny + ty - 1) / ty) * ((nz + kchunk - 1) / kchunk);
+}
 inline int auto_lane_cells_alone(int nx, int ny, int nz, int ty, int kchunk, size_t esz, int requested)
 {
     if (esz != 2) return 1;
     if (requested == 1 || requested == 2) return requested;
-    const double e1 = (double)nx / (64.0 * ((nx + 63) / 64)), e2 = (double)nx / (128.0 * ((nx + 127) / 128));
-    const long long tiles = (long long)((nx + 127) / 128) * ((ny + ty - 1) / ty) * ((nz + kchunk - 1) / kchunk);
-    return (e2 >= 0.95 * e1 && tiles >= 4096) ? 2 : 1;
+    if ((double)nx / (128.0 * ((nx + 127) / 128)) < 0.9) return 1;
+    return alone_blocks(nx, ny, nz, 128, ty, kchunk > 0 ? kchunk : 4) >= kAloneMinBlocks ? 2 : 1;
+}
+inline int auto_kchunk_alone(int nx, int ny, int nz, int ty, int width)
+{
+    for (int kc : {16, 8})
+        if (alone_blocks(nx, ny, nz, width, ty, kc) >= kAloneMinBlocks) return std::min(kc, nz);
+    return std::min(4, nz);
 }
 constexpr int kMurBlock = 256;            // Mur face cells per block
 

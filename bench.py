@@ -99,7 +99,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch shape by timing each candidate first (default: the library's "
-                         "default shape, 64x4 tiles of 4 planes, so the command's profile is reproducible)")
+                         "default shape for the grid, 64x4 tiles of 16 planes at 320x320x880, so the "
+                         "command's profile is reproducible)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--load-zero-terms", action="store_true",
                     help="fp32: load E^{n-1} and D^{n-1} on every row, also where the Debye update multiplies "
@@ -132,25 +133,25 @@ PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phanto
                                                           "fused_step_plain256_large_phantom.md")]
 
 
-def profiled_traffic(workload, storage, launch):
+def profiled_traffic(workload, storage, launch, loading=False):
     """(HBM bytes per fused launch, profile path) measured for this workload
     at this launch shape (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes,
-    scaled by a 1 GiB copy) in a committed profile, or (None, None)."""
+    scaled by a 1 GiB copy) in a committed profile, or (None, None).  The
+    kernel that loads E^{n-1} and D^{n-1} on every row (loading=True) has
+    its own sections ("... at <shape>, every row loading ...")."""
     import re
     if storage != "f32":
         return None, None
-    head = f"## {workload} at {launch}:"
+    head = f"## {workload} at {launch}, every row loading" if loading else f"## {workload} at {launch}:"
     for path in PROFILES:
         if not os.path.exists(path):
             continue
-        text = open(path).read()
-        if head not in text:
-            continue
-        sec = text.split(head, 1)[1].split("\n## ", 1)[0]
-        rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
-        wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
-        if rd and wr:
-            return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024, path
+        for sec in open(path).read().split(head)[1:]:
+            sec = sec.split("\n## ", 1)[0]
+            rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
+            wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
+            if rd and wr:
+                return (int(rd.group(1)) * int(rd.group(2)) + int(wr.group(1))) * 1024, path
     return None, None
 
 
@@ -293,10 +294,11 @@ def main():
     if a.traffic_kib:
         traffic, traffic_src = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024), "--traffic-kib"
     else:
-        traffic, prof = profiled_traffic(w.name, a.storage, launch)
+        traffic, prof = profiled_traffic(w.name, a.storage, launch, bool(a.load_zero_terms))
         traffic_src = (os.path.relpath(prof, ROOT) + f" (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
                        f"scale; {launch} launch)") if traffic is not None else \
-            f"no counter profile of {w.name} at launch {launch} under profiles/"
+            f"no counter profile of {w.name} at launch {launch}" + \
+            (" (every row loading)" if a.load_zero_terms else "") + " under profiles/"
     out = {
         "metric": "Gcell-updates/s (fused FDTD step, Debye media, Mur ABC)",
         "value": round(value, 3),

@@ -205,16 +205,17 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->ext = e;
     c->storage = e.storage;
     c->esz = fdtd_storage_bytes(e.storage);
-    // Defaults from launch-shape sweeps on MI355X.  fp32 (256^3 and
-    // 320x320x880 Debye phantoms, tile rows {4,8} x z-chunk {4,8,16} x three
-    // block orders): 64x4 tiles marching 4 planes, identity order, within
-    // 1-3% of every other shape at both sizes.  16-bit fields: 4 rows, two
-    // cells per lane (one 32-bit access per stream) and a deeper z-chunk
+    // Defaults from launch-shape sweeps on MI355X: 4 tile rows (8 rows gain
+    // 1-5% at 256^3 and lose at 320x320x880 and on small grids), identity
+    // block order; two cells per lane for 16-bit fields and the z-chunk
     // where those pay (auto_lane_cells_alone, auto_kchunk_alone).
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
     c->ty = e.tile_y ? e.tile_y : 4;
-    c->cpl = auto_lane_cells_alone(c->nx, c->ny, c->nz, c->ty, e.kchunk, c->esz, e.lane_cells);
-    if (e.kchunk) c->kchunk = std::min(e.kchunk, c->nz);
-    else c->kchunk = c->esz == 2 ? auto_kchunk_alone(c->nx, c->ny, c->nz, c->ty, 64 * c->cpl) : std::min(4, c->nz);
+    c->cpl = auto_lane_cells_alone(c->nx, c->ny, c->nz, c->ty, e.kchunk, c->esz, e.lane_cells, cus);
+    c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : auto_kchunk_alone(c->nx, c->ny, c->nz, c->ty, c->cpl, cus);
     set_shape(c, c->ty, c->kchunk, c->cpl, e.tile_x ? e.tile_x : 64);
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H

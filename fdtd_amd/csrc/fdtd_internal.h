@@ -236,38 +236,50 @@ inline int auto_lane_cells(int nx, size_t esz, int requested)
     return e2 >= 0.85 * e1 ? 2 : 1;
 }
 
-// Launch shape of a 16-bit grid launched alone (tile rows ty given), from a
-// sweep of 8 grid sizes (64^3 to 320x320x880) x {1, 2} cells per lane x
-// z-chunks {4, 8, 16} x rows {4, 8}, fp16 and bf16, on MI355X:
-//  * a launch needs about 1024 blocks to keep the GPU busy; beyond that,
-//    deeper z-chunks pay (fewer prologues): 320x320x880 runs 1101 us at 16
-//    planes against 1251 at 4, 256^3 pairs 203 against 221;
+// Default launch shape of a grid launched alone (tile rows ty given), from
+// sweeps on MI355X of 9 grid sizes (64^3 to 320x320x880) x tile rows {2, 4,
+// 8} x z-chunks {4, 8, 16}, fp32, and of 8 sizes x {1, 2} cells per lane,
+// fp16 and bf16:
 //  * pairs (128-cell tiles, one 32-bit access per stream) beat single cells
-//    where 128-cell rows are nearly full (256^3: 203 against 267 us; 128^3:
-//    35 against 45), and lose where they are not (320 wide: 1115 against
-//    1101; 200 wide: 53 against 48), also where 64-cell tiles would waste
-//    as many lanes.
-// So: pairs where 128-cell rows are at least 90% full and a pair launch
-// reaches 1024 blocks; then the deepest of 16, 8, 4 planes that keeps 1024
-// blocks (4 below that).  Every size of the sweep gets its best or
-// near-best shape (within 2%).  `kchunk` or `requested` > 0 fix that part.
-constexpr long long kAloneMinBlocks = 1024;
+//    where 128-cell rows are nearly full and the launch fills the GPU (fp16
+//    256^3: 203 against 267 us; 128^3: 35 against 45), and lose elsewhere
+//    (320 wide: 1115 against 1101; 200 wide: 53 against 48), also where
+//    64-cell tiles would waste as many lanes;
+//  * the z-chunk trades prologues (each block reloads and recomputes the
+//    plane below its chunk) against the partial last round of blocks.  A
+//    launch of B blocks on R resident ones costs about ceil(B / R) rounds
+//    of kchunk + 2 planes; the chunk of least cost among 4, 8 and 16 is the
+//    measured best, or within 2%, at every size but 200x100x100 (within 6%):
+//    fp32 320x320x880 16 planes, 1544 against 1620 us at 4; 192^3 8 planes,
+//    138 against 148 (16: 153); fp16 320x320x880 16 planes, 1101 against 1251.
+// Resident blocks per CU: 4 waves per SIMD for pairs (their VGPR budget), 5
+// for single cells, over ty waves a block.
 inline long long alone_blocks(int nx, int ny, int nz, int width, int ty, int kchunk)
 {
     return (long long)((nx + width - 1) / width) * ((ny + ty - 1) / ty) * ((nz + kchunk - 1) / kchunk);
 }
-inline int auto_lane_cells_alone(int nx, int ny, int nz, int ty, int kchunk, size_t esz, int requested)
+inline long long alone_resident(int cus, int ty, int cpl)
+{
+    return (long long)cus * std::max(1, 4 * (cpl == 2 ? 4 : 5) / ty);
+}
+inline int auto_lane_cells_alone(int nx, int ny, int nz, int ty, int kchunk, size_t esz, int requested, int cus)
 {
     if (esz != 2) return 1;
     if (requested == 1 || requested == 2) return requested;
     if ((double)nx / (128.0 * ((nx + 127) / 128)) < 0.9) return 1;
-    return alone_blocks(nx, ny, nz, 128, ty, kchunk > 0 ? kchunk : 4) >= kAloneMinBlocks ? 2 : 1;
+    return alone_blocks(nx, ny, nz, 128, ty, kchunk > 0 ? kchunk : 4) >= alone_resident(cus, ty, 2) ? 2 : 1;
 }
-inline int auto_kchunk_alone(int nx, int ny, int nz, int ty, int width)
+inline int auto_kchunk_alone(int nx, int ny, int nz, int ty, int cpl, int cus)
 {
-    for (int kc : {16, 8})
-        if (alone_blocks(nx, ny, nz, width, ty, kc) >= kAloneMinBlocks) return std::min(kc, nz);
-    return std::min(4, nz);
+    const long long R = alone_resident(cus, ty, cpl);
+    int best = 4;
+    long long best_cost = -1;
+    for (int kc : {4, 8, 16}) {
+        const long long rounds = (alone_blocks(nx, ny, nz, 64 * cpl, ty, kc) + R - 1) / R;
+        const long long cost = rounds * (kc + 2);
+        if (best_cost < 0 || cost <= best_cost) { best = kc; best_cost = cost; }
+    }
+    return std::min(best, nz);
 }
 constexpr int kMurBlock = 256;            // Mur face cells per block
 

@@ -21,16 +21,24 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_profiled_traffic_matches_the_launch_shape():
-    # the autotuned 256^3 launch and the library default both have counters
-    t, p = bench.profiled_traffic("plain_256", "f32", "128x2x6")
-    t0, _ = bench.profiled_traffic("plain_256", "f32", "64x4x4")
+    """Each figure comes from the section of its own workload, launch shape and
+    kernel form (zero-term rows, or every row loading), wherever it sits."""
+    alg = 97 * 320 * 320 * 880
+    # the headline workload at the library's default launch shape has its own counter passes
+    tl, pl = bench.profiled_traffic("large_phantom", "f32", "64x4x16")
+    assert tl and 1.0 < tl / alg < 1.8 and pl.endswith("fused_step_large_phantom.md")
+    # 4-plane chunks read more (a z-chunk prologue per 4 planes), the loading kernel more still
+    t4, _ = bench.profiled_traffic("large_phantom", "f32", "64x4x4")
+    t4l, _ = bench.profiled_traffic("large_phantom", "f32", "64x4x4", loading=True)
+    assert t4 and t4l and tl < t4 < t4l < 1.8 * alg
+    # the 256^3 counters were taken with the loading kernel only
+    t, p = bench.profiled_traffic("plain_256", "f32", "128x2x6", loading=True)
+    t0, _ = bench.profiled_traffic("plain_256", "f32", "64x4x4", loading=True)
     assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t and p.endswith(".md")
-    # the headline workload at the default launch shape has its own counter passes
-    tl, pl = bench.profiled_traffic("large_phantom", "f32", "64x4x4")
-    assert tl and 1.0 < tl / (97 * 320 * 320 * 880) < 1.8
+    assert bench.profiled_traffic("plain_256", "f32", "128x2x6") == (None, None)
     # a shape without counters, or 16-bit storage, gives none
     assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
-    assert bench.profiled_traffic("plain_256", "f16", "128x2x6") == (None, None)
+    assert bench.profiled_traffic("plain_256", "f16", "128x2x6", loading=True) == (None, None)
 
 
 def test_headline_defaults():

@@ -82,13 +82,14 @@ def test_ctx_autotune_restores_state(storage):
 
 @pytest.mark.parametrize("storage,shape,tile_x,kchunk", [
     (F.STORE_F16, (256, 256, 256), 128, 16), (F.STORE_BF16, (320, 320, 880), 64, 16), (F.STORE_F16, (96, 96, 96), 64, 4),
-    (F.STORE_F16, (200, 100, 100), 64, 8), (F.STORE_BF16, (128, 128, 128), 128, 4), (F.STORE_F16, (64, 64, 64), 64, 4),
-    (F.STORE_F32, (320, 320, 880), 64, 4), (F.STORE_F32, (96, 96, 96), 64, 4)])
+    (F.STORE_F16, (200, 100, 100), 64, 4), (F.STORE_BF16, (128, 128, 128), 128, 4), (F.STORE_F16, (64, 64, 64), 64, 4),
+    (F.STORE_F32, (320, 320, 880), 64, 16), (F.STORE_F32, (256, 256, 256), 64, 8), (F.STORE_F32, (192, 192, 192), 64, 8),
+    (F.STORE_F32, (96, 96, 96), 64, 4)])
 def test_default_launch_shapes(storage, shape, tile_x, kchunk):
-    """Default shapes of a context: 16-bit grids take pairs where 128-cell rows
-    are at least 90% full and a pair launch reaches 1024 blocks, then the
-    deepest of 16, 8, 4 planes that keeps 1024 blocks; fp32 keeps 64x4 tiles
-    of 4 planes."""
+    """Default shapes of a context (4 tile rows): 16-bit grids take pairs where
+    128-cell rows are at least 90% full and a pair launch fills the GPU's
+    resident blocks; the z-chunk is the one of least cost in rounds of
+    resident blocks times (planes + 2)."""
     with F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0), storage=storage) as ctx:
         s = ctx.launch_shape()
         assert (s["tile_x"], s["tile_y"], s["kchunk"]) == (tile_x, 4, kchunk), s

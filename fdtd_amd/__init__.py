@@ -16,6 +16,7 @@ goes to the HIP kernels and fails loudly when the library or a GPU is missing.
 """
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
@@ -45,12 +46,14 @@ EXPORTS = {
                    "fdtd_ctx_set_materials", "fdtd_ctx_set_material_arrays", "fdtd_ctx_set_rp",
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
-                   "fdtd_ctx_steps_done", "fdtd_ctx_snapshot", "fdtd_probe_read", "fdtd_slices_read", "fdtd_ctx_launch_shape",
+                   "fdtd_ctx_steps_done", "fdtd_ctx_snapshot", "fdtd_probe_read", "fdtd_probe_peak_units", "fdtd_slices_read",
+                   "fdtd_ctx_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
                    "fdtd_ctx_autotune", "fdtd_ctx_halo_bytes", "fdtd_ctx_halo_pack", "fdtd_ctx_halo_unpack",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
                    "fdtd_batch_remove", "fdtd_batch_run", "fdtd_batch_status", "fdtd_batch_probe_read",
+                   "fdtd_batch_probe_peak_units",
                    "fdtd_batch_get_field", "fdtd_batch_set_shape", "fdtd_batch_autotune", "fdtd_batch_info",
                    "fdtd_dropin_solve", "fdtd_dropin_solve_noalloc", "fdtd_stream_probe", "fdtd_stream_probe_layout", "fdtd_load_tissue_table", "fdtd_load_phantom_ids",
                    "fdtd_serve_create", "fdtd_serve_destroy", "fdtd_serve_submit", "fdtd_serve_wait",
@@ -165,6 +168,7 @@ def lib():
         L.fdtd_ctx_steps_done.argtypes = [vp]
         L.fdtd_ctx_snapshot.argtypes = [vp, vp, vp]
         L.fdtd_probe_read.argtypes = [vp, vp, i]
+        L.fdtd_probe_peak_units.argtypes = [vp, vp, i]
         L.fdtd_slices_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
         L.fdtd_ctx_set_timing.argtypes = [vp, i, i]
@@ -188,6 +192,7 @@ def lib():
         L.fdtd_batch_run.argtypes = [vp, i, vp]
         L.fdtd_batch_status.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.fdtd_batch_probe_read.argtypes = [vp, i, vp, i]
+        L.fdtd_batch_probe_peak_units.argtypes = [vp, i, vp, i]
         L.fdtd_batch_get_field.argtypes = [vp, i, i, i, vp]
         L.fdtd_batch_set_shape.argtypes = [vp, i, i, i]
         L.fdtd_batch_autotune.argtypes = [vp, i, vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -226,6 +231,19 @@ def lib():
         L.writeResults3d.argtypes = [C.POINTER(FdtdParams)] + [vp] * 9 + [i]
         _lib = L
     return _lib
+
+
+class PrecisionWarning(UserWarning):
+    """A 16-bit result lost the precision of its storage format."""
+
+
+def _warn_low_probes(peak, low, where=""):
+    if low > 0:
+        idx = [int(p) for p in np.nonzero((peak > 0) & (peak < 2.0 ** -14))[0]]
+        warnings.warn(f"{where}fp16 storage: probe(s) {idx} peaked below fp16's normal range (2^-14 stored units; "
+                      f"peaks {[float(f'{peak[p]:.3g}') for p in idx]}): their fields never had fp16's 11 "
+                      f"significant bits and may be far off; one scale per field cannot hold the sources' near "
+                      f"field and these probes at once -- use fp32 storage", PrecisionWarning, stacklevel=3)
 
 
 def check(rc, what=""):
@@ -273,6 +291,7 @@ class Context:
         self._h = h
         self.shape = (params.nx, params.ny, params.nz)
         self.nprobe = 0
+        self.storage = storage
 
     def close(self):
         if getattr(self, "_h", None):
@@ -366,12 +385,26 @@ class Context:
         return lib().fdtd_ctx_steps_done(self._h)
 
     def probes(self, max_rows=None):
+        """Recorded probe rows (SI units).  Under fp16 storage, warns when a
+        probe has so far peaked below fp16's normal range (probe_peak_units)."""
         rows = self.n if max_rows is None else max_rows
         out = np.zeros((max(rows, 1), max(self.nprobe, 1)), np.float64)
         got = lib().fdtd_probe_read(self._h, _ptr(out), rows)
         if got < 0:
             check(got, "probe_read")
+        if self.storage == STORE_F16 and self.nprobe and got > 0:
+            _warn_low_probes(*self.probe_peak_units())
         return out[:got, :self.nprobe]
+
+    def probe_peak_units(self):
+        """(peak |value| of each probe so far in stored units, how many saw a
+        signal that peaked below 2^-14, fp16's smallest normal number, under
+        fp16 storage)."""
+        peak = np.zeros(max(self.nprobe, 1), np.float64)
+        low = lib().fdtd_probe_peak_units(self._h, _ptr(peak), self.nprobe)
+        if low < 0:
+            check(low, "probe_peak_units")
+        return peak[:self.nprobe], low
 
     def snapshot(self, dev_out, stream=None):
         """Enqueue the nine newest fields, packed fp32 in SI units, into the
@@ -578,13 +611,26 @@ class Batch:
         return d.value, t.value
 
     def probes(self, i):
+        """Grid i's recorded probe rows (SI units); warns as Context.probes
+        does under fp16 storage."""
         inst = self._inst[i]
         rows = inst.steps
         out = np.zeros((max(rows, 1), max(inst.nprobe, 1)), np.float64)
         got = lib().fdtd_batch_probe_read(self._h, i, _ptr(out), rows)
         if got < 0:
             check(got, "fdtd_batch_probe_read")
+        if inst.ext.storage == STORE_F16 and inst.nprobe and got > 0:
+            _warn_low_probes(*self.probe_peak_units(i), f"batch grid {i}: ")
         return out[:got, :inst.nprobe]
+
+    def probe_peak_units(self, i):
+        """Context.probe_peak_units for grid i."""
+        n = self._inst[i].nprobe
+        peak = np.zeros(max(n, 1), np.float64)
+        low = lib().fdtd_batch_probe_peak_units(self._h, i, _ptr(peak), n)
+        if low < 0:
+            check(low, "fdtd_batch_probe_peak_units")
+        return peak[:n], low
 
     def get(self, i, field, level=0):
         fid = FIELDS.index(field) if isinstance(field, str) else field

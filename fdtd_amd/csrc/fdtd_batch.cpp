@@ -180,6 +180,7 @@ struct fdtd_batch {
         int steps = 0;
         double sE = 1, sD = 1, sH = 1;
         float mu0 = 0, eps0 = 0;
+        std::vector<int> probe_field;  // field of each probe
     };
     std::vector<Inst> in;
     long long step = 0;
@@ -436,6 +437,8 @@ extern "C" int fdtd_batch_add(fdtd_batch *b, const FdtdParams *p, const FdtdExt 
     I.steps = in->steps;
     I.mu0 = p->mu0;
     I.eps0 = p->eps0;
+    I.probe_field.resize(in->nprobe);
+    for (int q = 0; q < in->nprobe; q++) I.probe_field[q] = in->probes[4 * q];
     if (e.storage != kStoreF32) fdtd_auto_scales(in->jz, in->jz_len, p->eps0, e, &I.sE, &I.sD, &I.sH);
     else I.sE = I.sD = I.sH = 1.0;
 
@@ -603,6 +606,19 @@ extern "C" int fdtd_batch_probe_read(fdtd_batch *b, int id, double *host, int ma
     if (rows > 0 && I.d.nprobe > 0)
         HIP_TRY(hipMemcpy(host, I.d.probe_out, sizeof(double) * (size_t)rows * I.d.nprobe, hipMemcpyDeviceToHost));
     return I.d.nprobe > 0 ? rows : 0;
+}
+
+extern "C" int fdtd_batch_probe_peak_units(fdtd_batch *b, int id, double *peak, int nprobe)
+{
+    if (!b || id < 0 || id >= (int)b->in.size() || !b->in[id].used) return fdtd_fail(FDTD_EINVAL, "bad instance id %d", id);
+    const auto &I = b->in[id];
+    if (nprobe < 0 || (nprobe > 0 && !peak)) return fdtd_fail(FDTD_EINVAL, "bad probe peak query");
+    if (nprobe != I.d.nprobe) return fdtd_fail(FDTD_EINVAL, "%d probes asked, the grid records %d", nprobe, I.d.nprobe);
+    if (nprobe == 0) return 0;
+    std::vector<double> series((size_t)I.steps * nprobe);
+    const int rows = fdtd_batch_probe_read(b, id, series.data(), I.steps);
+    if (rows < 0) return rows;
+    return fdtd_probe_peaks(I.d.storage, series.data(), rows, nprobe, I.probe_field.data(), I.d.factor, peak);
 }
 
 extern "C" int fdtd_batch_get_field(fdtd_batch *b, int id, int field, int level, float *host)

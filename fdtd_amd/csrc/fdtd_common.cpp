@@ -280,6 +280,21 @@ int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes)
     return 0;
 }
 
+int fdtd_probe_peaks(int storage, const double *series, int rows, int nprobe, const int *fields, const double *factor,
+                     double *peak)
+{
+    const double f16_min_normal = 6.103515625e-05;   // 2^-14
+    int low = 0;
+    for (int p = 0; p < nprobe; p++) {
+        double m = 0.0;
+        for (int r = 0; r < rows; r++) m = std::max(m, std::fabs(series[(size_t)r * nprobe + p]));
+        const double f = std::fabs(factor[fields[p]]);
+        peak[p] = f > 0.0 ? m / f : m;
+        if (storage == kStoreF16 && peak[p] > 0.0 && peak[p] < f16_min_normal) low++;
+    }
+    return low;
+}
+
 void fdtd_to_storage_pitched(int storage, const float *in, int nx, size_t rows, int pitch, double factor, void *out)
 {
     const size_t esz = fdtd_storage_bytes(storage);

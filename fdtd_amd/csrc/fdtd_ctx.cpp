@@ -56,6 +56,7 @@ struct fdtd_ctx {
     float4 *murx = nullptr;              // x-face Mur coefficients: 2 planes of ny*nz
     std::vector<unsigned char> ids_host;  // material ids without source bits
     std::vector<int> src;                 // 3 per source
+    std::vector<int> probe_field;         // field of each probe
     std::vector<float> jz;
     double sE = 1.0, sD = 1.0, sH = 1.0;  // 16-bit storage scales
     bool scales_set = false;
@@ -375,6 +376,8 @@ extern "C" int fdtd_ctx_set_probes(fdtd_ctx *c, int nprobe, const int *probes, i
     HIP_TRY(hipMemcpy(c->probes, probes, sizeof(int) * 4 * nprobe, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->probe_out, sizeof(double) * (size_t)nprobe * capacity));
     HIP_TRY(hipMemset(c->probe_out, 0, sizeof(double) * (size_t)nprobe * capacity));
+    c->probe_field.resize(nprobe);
+    for (int p = 0; p < nprobe; p++) c->probe_field[p] = probes[4 * p];
     c->nprobe = nprobe;
     c->probe_cap = capacity;
     return 0;
@@ -569,6 +572,21 @@ extern "C" int fdtd_probe_read(fdtd_ctx *c, double *host, int max_rows)
     if (rows > 0 && c->nprobe > 0)
         HIP_TRY(hipMemcpy(host, c->probe_out, sizeof(double) * (size_t)rows * c->nprobe, hipMemcpyDeviceToHost));
     return c->nprobe > 0 ? rows : 0;
+}
+
+extern "C" int fdtd_probe_peak_units(fdtd_ctx *c, double *peak, int nprobe)
+{
+    if (!c || nprobe < 0 || (nprobe > 0 && !peak)) return fdtd_fail(FDTD_EINVAL, "bad probe peak query");
+    if (nprobe != c->nprobe) return fdtd_fail(FDTD_EINVAL, "%d probes asked, the context records %d", nprobe, c->nprobe);
+    if (nprobe == 0) return 0;
+    HIP_TRY(hipDeviceSynchronize());
+    const int rows = c->probe_rows;
+    std::vector<double> series((size_t)rows * nprobe);
+    if (rows > 0) HIP_TRY(hipMemcpy(series.data(), c->probe_out, sizeof(double) * series.size(), hipMemcpyDeviceToHost));
+    resolve_scales(c);
+    double factor[9];
+    for (int f = 0; f < 9; f++) factor[f] = field_factor(c, f);
+    return fdtd_probe_peaks(c->storage, series.data(), rows, nprobe, c->probe_field.data(), factor, peak);
 }
 
 extern "C" int fdtd_slices_read(fdtd_ctx *c, int *steps, float *data, int max_slices)

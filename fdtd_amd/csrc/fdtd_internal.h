@@ -408,6 +408,13 @@ int dropin_check(const FdtdParams *p, const FdtdExt *ext, const FdtdInstance *in
 int dropin_run(const FdtdParams *p, const FdtdExt &e, const FdtdInstance *in, const DropinState &st,
                double *probes_host, hipStream_t stream);
 int fdtd_check_probes(int nx, int ny, int nz, int nprobe, const int *probes);
+// Largest |value| of each probe over `rows` rows of a series (SI units, row
+// major, nprobe per row), in stored units (divided by its field's storage
+// factor).  Returns how many probes saw a signal (peak > 0) that peaked
+// below fp16's smallest normal number under fp16 storage (0 for the other
+// storages).
+int fdtd_probe_peaks(int storage, const double *series, int rows, int nprobe, const int *fields, const double *factor,
+                     double *peak);
 
 #define HIP_TRY(call)                                              \
     do {                                                           \

@@ -192,6 +192,16 @@ int fdtd_ctx_snapshot(fdtd_ctx *ctx, float *dev_out, hipStream_t stream);
 /* Copies up to max_rows recorded probe rows to host (synchronises the
  * context's streams); returns the number of rows or a negative code. */
 int fdtd_probe_read(fdtd_ctx *ctx, double *host, int max_rows);
+/* Precision check for 16-bit storage (no counterpart in the reference, whose
+ * fields are fp32).  peak[p] = the largest |value| probe p has recorded so
+ * far, in stored units (the SI value over its field's scale; SI for fp32
+ * storage); nprobe must equal the context's probe count.  Returns how many
+ * probes have seen a signal that peaked below 2^-14, fp16's smallest normal
+ * number, under fp16 storage (0 for the others), or a negative code.  Such a probe's field
+ * never had fp16's 11 significant bits: one scale per field cannot hold
+ * both a strong source's near field and what it observes there, and its
+ * error against FP64 can reach O(1) (fp32 storage is the remedy). */
+int fdtd_probe_peak_units(fdtd_ctx *ctx, double *peak, int nprobe);
 /* Waits for the slice copies; returns the number captured and fills
  * steps[] (the step count after which each slice was taken) and data
  * (slices x 9 fields x nx*ny floats) up to max_slices. */
@@ -312,6 +322,8 @@ int fdtd_batch_run(fdtd_batch *b, int steps, hipStream_t stream);
 int fdtd_batch_status(fdtd_batch *b, int id, int *steps_done, int *steps_total);
 /* Waits for the device; copies up to max_rows probe rows; returns rows. */
 int fdtd_batch_probe_read(fdtd_batch *b, int id, double *host, int max_rows);
+/* fdtd_probe_peak_units for grid `id` of a batch. */
+int fdtd_batch_probe_peak_units(fdtd_batch *b, int id, double *peak, int nprobe);
 /* Waits for the device; level as fdtd_ctx_get_field, after the grid's
  * steps so far. */
 int fdtd_batch_get_field(fdtd_batch *b, int id, int field, int level, float *host);

@@ -12,7 +12,10 @@ import pytest
 import fdtd_amd as F
 from tests.helpers import EPS0, MU0, bits_equal, first_diff, random_materials, random_rp, random_sources
 
-pytestmark = pytest.mark.gpu
+# fp16 grids here run a few dozen steps with probes far from their sources, whose precursor
+# peaks below fp16's normal range: the precision warning is expected and not what these
+# bit-for-bit comparisons test
+pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore::fdtd_amd.PrecisionWarning")]
 
 DX = 2e-3
 DT = 0.99 * DX / (3.0e8 * np.sqrt(3.0))

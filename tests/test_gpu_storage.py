@@ -233,3 +233,63 @@ def test_16bit_face_probes_read_the_stored_fields(storage, shape, kchunk, lane_c
                 abs(row[p] - want) <= 1e-6 * abs(want) + 1e-37
             assert same, f"step {n + 1} probe {p} {O.FIELDS[f]} ({i},{j},{k}): {row[p]} vs {want}"
     ctx.close()
+
+
+def _peaks(w, storage, steps, scales=(0.0, 0.0, 0.0)):
+    ctx = F.Context(w.params, storage=storage, scales=scales)
+    ctx.set_materials(w.ids, w.table)
+    ctx.set_sources(w.sources, w.jz)
+    ctx.set_probes(w.probes, steps)
+    ctx.run(steps)
+    return ctx
+
+
+def test_fp16_precision_check():
+    """fdtd_probe_peak_units flags probes whose signal peaked below fp16's
+    normal range (2^-14 stored units): with every field's scale 3e4 times the
+    automatic one, plain_small's centre probe peaks near 1.5e-5 units and is
+    flagged (with a PrecisionWarning from probes()); at the automatic scales
+    it peaks near 0.45 units and is not.  A batch grid reports the same
+    peaks, and fp32 storage never flags."""
+    w = W.plain_small(steps=300)
+    ctx = _peaks(w, F.STORE_F16, 300)
+    auto = ctx.scales()
+    peak, low = ctx.probe_peak_units()
+    assert low == 0 and (peak >= 2.0 ** -14).all()
+    ctx.close()
+    big = tuple(s * 3e4 for s in auto)
+    ctx = _peaks(w, F.STORE_F16, 300, big)
+    peak_s, low_s = ctx.probe_peak_units()
+    print(f"\nplain_small fp16 peaks at the automatic scales {peak}, at 3e4 times them {peak_s}", flush=True)
+    assert low_s == len(peak_s) and ((peak_s > 0) & (peak_s < 2.0 ** -14)).all()
+    with pytest.warns(F.PrecisionWarning, match="peaked below fp16's normal range"):
+        ctx.probes()
+    ctx.close()
+    inst = F.Instance(w.params, F.make_ext(storage=F.STORE_F16, scales=big), w.ids, w.table, w.sources, w.jz,
+                      300, w.probes)
+    with F.Batch(int(inst.nbytes() * 1.05) + (32 << 20)) as b:
+        i = b.add(inst)
+        b.run(300)
+        pb, lb = b.probe_peak_units(i)
+        assert lb == low_s and np.array_equal(pb, peak_s)
+        with pytest.warns(F.PrecisionWarning):
+            b.probes(i)
+    ctx = _peaks(w, F.STORE_F32, 300)
+    assert ctx.probe_peak_units()[1] == 0
+    ctx.close()
+
+
+def test_fp16_precision_check_on_the_strong_dipole():
+    """The case where fp16 storage fails (320x320x880: the 10-cell dipole's
+    field is ~1e11 times the body centre's): after 2000 steps the centre Ez
+    probe and the Ex probe at z = 220 peak near 5e-6 stored units (their
+    true fields ~1e-6 and ~4e-7: the fp16 values are noise) and are flagged;
+    the probe 8 cells from the dipole peaks near 1.4 units and is not."""
+    w = W.large_phantom(steps=2000)
+    ctx = _peaks(w, F.STORE_F16, 2000)
+    peak, low = ctx.probe_peak_units()
+    print(f"\nlarge_phantom fp16 probe peaks after 2000 steps: {peak} stored units", flush=True)
+    assert low == 2 and 0 < peak[0] < 2.0 ** -14 and 0 < peak[2] < 2.0 ** -14 and peak[1] > 0.5
+    with pytest.warns(F.PrecisionWarning, match=r"probe\(s\) \[0, 2\]"):
+        ctx.probes()
+    ctx.close()

@@ -129,7 +129,7 @@ def build_workload(name, steps, a=None):
     return W.plain_small(steps=steps)
 
 
-PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phantom.md",
+PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phantom.md", "fused_step_plain_256.md",
                                                           "fused_step_plain256_large_phantom.md")]
 
 

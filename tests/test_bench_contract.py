@@ -36,6 +36,9 @@ def test_profiled_traffic_matches_the_launch_shape():
     t0, _ = bench.profiled_traffic("plain_256", "f32", "64x4x4", loading=True)
     assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t and p.endswith(".md")
     assert bench.profiled_traffic("plain_256", "f32", "128x2x6") == (None, None)
+    # 256^3 at its default shape, with zero-term rows
+    t8, p8 = bench.profiled_traffic("plain_256", "f32", "64x4x8")
+    assert t8 and 1.2 < t8 / (97 * 256 ** 3) < 1.7 and p8.endswith("fused_step_plain_256.md")
     # a shape without counters, or 16-bit storage, gives none
     assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
     assert bench.profiled_traffic("plain_256", "f16", "128x2x6", loading=True) == (None, None)

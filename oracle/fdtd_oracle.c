@@ -37,7 +37,11 @@
  *   or_load_specs/pgm     the tissue table fixture (53 rows) and a PGM file in
  *                         the reference's layout.
  * Parity unpinned: nvcc's exact FMA contraction on the reference device code
- * (no NVIDIA GPU or nvcc exists here); the reference's undefined behaviour
+ * (no NVIDIA GPU or nvcc exists here); which rp plane is the face (p0) and
+ * which the neighbour (p1) in or_update_mur_*, and where the FP64
+ * promotions sit in or_update_e_f32 (both as read in the reference's text:
+ * with rp uniform a swap is invisible, and a moved promotion changes results
+ * by ulps, inside every FP64 bound); the reference's undefined behaviour
  * (out-of-range coefficient loads, specs[94], dJz past 512) is replaced by
  * defined behaviour: no out-of-range loads, code 94 stored as parsed, and
  * jz = 0 past the end of the table.

@@ -581,9 +581,12 @@ extern "C" int fdtd_probe_peak_units(fdtd_ctx *c, double *peak, int nprobe)
     if (nprobe == 0) return 0;
     HIP_TRY(hipDeviceSynchronize());
     const int rows = c->probe_rows;
+    if (rows == 0) {   // nothing recorded (and the scales may not be fixed before the first step)
+        std::fill(peak, peak + nprobe, 0.0);
+        return 0;
+    }
     std::vector<double> series((size_t)rows * nprobe);
-    if (rows > 0) HIP_TRY(hipMemcpy(series.data(), c->probe_out, sizeof(double) * series.size(), hipMemcpyDeviceToHost));
-    resolve_scales(c);
+    HIP_TRY(hipMemcpy(series.data(), c->probe_out, sizeof(double) * series.size(), hipMemcpyDeviceToHost));
     double factor[9];
     for (int f = 0; f < 9; f++) factor[f] = field_factor(c, f);
     return fdtd_probe_peaks(c->storage, series.data(), rows, nprobe, c->probe_field.data(), factor, peak);

@@ -257,6 +257,16 @@ def test_fp16_precision_check():
     peak, low = ctx.probe_peak_units()
     assert low == 0 and (peak >= 2.0 ** -14).all()
     ctx.close()
+    # before any step: zeros, and the automatic scales are still taken from the waveform set later
+    c0 = F.Context(w.params, storage=F.STORE_F16)
+    c0.set_probes(w.probes, 10)
+    p0, l0 = c0.probe_peak_units()
+    assert l0 == 0 and not p0.any()
+    c0.set_materials(w.ids, w.table)
+    c0.set_sources(w.sources, w.jz)
+    c0.run(1)
+    assert c0.scales() == auto
+    c0.close()
     big = tuple(s * 3e4 for s in auto)
     ctx = _peaks(w, F.STORE_F16, 300, big)
     peak_s, low_s = ctx.probe_peak_units()

@@ -295,12 +295,17 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         const uint32_t ob = (uint32_t)k * szb + c_own;
         o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
         o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
-        if (ZS && zs) {
-            o.dxm = o.dym = o.dzm = o.exm = o.eym = o.ezm = 0.0f;
-        } else {
-            o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
-            o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+        // Loaded into locals and assigned to the struct unconditionally: with the
+        // members assigned on both branches, the compiler merged the two paths'
+        // last stores into one through a selected member address and kept part
+        // of the struct in scratch memory, written back to HBM every plane.
+        float dxm = 0.0f, dym = 0.0f, dzm = 0.0f, exm = 0.0f, eym = 0.0f, ezm = 0.0f;
+        if (!(ZS && zs)) {
+            dxm = X::ld_nt(dx1, ob); dym = X::ld_nt(dy1, ob); dzm = X::ld_nt(dz1, ob);
+            exm = X::ld_nt(ex1, ob); eym = X::ld_nt(ey1, ob); ezm = X::ld_nt(ez1, ob);
         }
+        o.dxm = dxm; o.dym = dym; o.dzm = dzm;
+        o.exm = exm; o.eym = eym; o.ezm = ezm;
         if (load_id) o.id = __builtin_nontemporal_load(ids + ob / ES);
         o.zs = ZS && zs;
         // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TXW, row cr): Hy, Hz

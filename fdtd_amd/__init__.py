@@ -47,7 +47,7 @@ EXPORTS = {
                    "fdtd_ctx_set_sources", "fdtd_ctx_set_probes", "fdtd_ctx_set_slices", "fdtd_ctx_set_field",
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_ctx_snapshot", "fdtd_probe_read", "fdtd_probe_peak_units", "fdtd_slices_read",
-                   "fdtd_ctx_launch_shape",
+                   "fdtd_ctx_launch_shape", "fdtd_default_launch_shape",
                    "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
                    "fdtd_ctx_autotune", "fdtd_ctx_halo_bytes", "fdtd_ctx_halo_pack", "fdtd_ctx_halo_unpack",
@@ -171,6 +171,7 @@ def lib():
         L.fdtd_probe_peak_units.argtypes = [vp, vp, i]
         L.fdtd_slices_read.argtypes = [vp, vp, vp, i]
         L.fdtd_ctx_launch_shape.argtypes = [vp, vp, vp, vp, vp]
+        L.fdtd_default_launch_shape.argtypes = [C.POINTER(FdtdParams), C.POINTER(FdtdExt), i, vp, vp, vp, vp]
         L.fdtd_ctx_set_timing.argtypes = [vp, i, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
@@ -472,6 +473,19 @@ def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, 
     ext.scale_e, ext.scale_d, ext.scale_h = scales
     ext.load_zero_terms = load_zero_terms
     return ext
+
+
+def default_launch_shape(params, storage=STORE_F32, cus=0, **ext_fields):
+    """The launch shape a Context would take for this grid (no device memory
+    is touched): dict(tile_x, tile_y, kchunk, lane_cells).  cus = compute
+    units of the target GPU (0: the current device's, 256 without one)."""
+    ext = make_ext(storage=storage)
+    for k, v in ext_fields.items():
+        setattr(ext, k, v)
+    v = [C.c_int() for _ in range(4)]
+    check(lib().fdtd_default_launch_shape(C.byref(params), C.byref(ext), cus, *[C.byref(x) for x in v]),
+          "fdtd_default_launch_shape")
+    return dict(tile_x=v[0].value, tile_y=v[1].value, kchunk=v[2].value, lane_cells=v[3].value)
 
 
 class Instance:

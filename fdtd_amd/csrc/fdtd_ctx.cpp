@@ -152,6 +152,28 @@ void set_shape(fdtd_ctx *c, int ty, int kchunk, int cpl, int txw = 64)
     c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
 }
 
+// Launch shape a context takes for a grid, from launch-shape sweeps on
+// MI355X: 4 tile rows (8 rows gain 1-5% at 256^3 and lose at 320x320x880 and
+// on small grids), identity block order; two cells per lane for 16-bit
+// fields and the z-chunk where those pay (auto_lane_cells_alone,
+// auto_kchunk_alone).  FdtdExt's tile_y, kchunk, lane_cells and tile_x fix
+// their parts.  cus = 0: the current device's compute units (256 when none
+// can be queried).
+void default_shape(int nx, int ny, int nz, const FdtdExt &e, int cus, int *ty, int *kchunk, int *cpl, int *txw)
+{
+    if (cus <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const size_t esz = fdtd_storage_bytes(e.storage);
+    *ty = e.tile_y ? e.tile_y : 4;
+    *cpl = auto_lane_cells_alone(nx, ny, nz, *ty, e.kchunk, esz, e.lane_cells, cus);
+    *kchunk = e.kchunk ? std::min(e.kchunk, nz) : auto_kchunk_alone(nx, ny, nz, *ty, *cpl, cus);
+    *txw = *cpl == 2 ? 64 : (e.tile_x ? e.tile_x : 64);
+}
+
 hipError_t upload_murx(fdtd_ctx *c, const float *rpx0, const float *rpxEnd)
 {
     const size_t n = (size_t)c->ny * c->nz;
@@ -176,6 +198,24 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->lane_cells = 0;
     ext->tile_x = 0;
     ext->load_zero_terms = 0;
+    return 0;
+}
+
+extern "C" int fdtd_default_launch_shape(const FdtdParams *p, const FdtdExt *ext, int cus, int *tile_x, int *tile_y,
+                                         int *kchunk, int *lane_cells)
+{
+    if (!p) return fdtd_fail(FDTD_EINVAL, "null argument");
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fdtd_fail(FDTD_ERANGE, "grid %dx%dx%d: each edge must be >= 3", p->nx, p->ny, p->nz);
+    FdtdExt e;
+    fdtd_ext_defaults(&e);
+    if (ext) e = *ext;
+    if (int rc = fdtd_check_ext(e)) return rc;
+    int ty = 0, kc = 0, cpl = 0, txw = 0;
+    default_shape(p->nx, p->ny, p->nz, e, cus, &ty, &kc, &cpl, &txw);
+    if (tile_x) *tile_x = cpl == 2 ? 128 : txw;
+    if (tile_y) *tile_y = ty;
+    if (kchunk) *kchunk = kc;
+    if (lane_cells) *lane_cells = cpl;
     return 0;
 }
 
@@ -206,18 +246,9 @@ extern "C" int fdtd_ctx_create(const FdtdParams *p, const FdtdExt *ext, fdtd_ctx
     c->ext = e;
     c->storage = e.storage;
     c->esz = fdtd_storage_bytes(e.storage);
-    // Defaults from launch-shape sweeps on MI355X: 4 tile rows (8 rows gain
-    // 1-5% at 256^3 and lose at 320x320x880 and on small grids), identity
-    // block order; two cells per lane for 16-bit fields and the z-chunk
-    // where those pay (auto_lane_cells_alone, auto_kchunk_alone).
-    int cus = 0, dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    c->ty = e.tile_y ? e.tile_y : 4;
-    c->cpl = auto_lane_cells_alone(c->nx, c->ny, c->nz, c->ty, e.kchunk, c->esz, e.lane_cells, cus);
-    c->kchunk = e.kchunk ? std::min(e.kchunk, c->nz) : auto_kchunk_alone(c->nx, c->ny, c->nz, c->ty, c->cpl, cus);
-    set_shape(c, c->ty, c->kchunk, c->cpl, e.tile_x ? e.tile_x : 64);
+    int txw = 64;
+    default_shape(c->nx, c->ny, c->nz, e, 0, &c->ty, &c->kchunk, &c->cpl, &txw);
+    set_shape(c, c->ty, c->kchunk, c->cpl, txw);
 
     const size_t nf = 24;  // 9 E + 9 D + 6 H
     c->guard = fused_guard(c->pitch, c->plane);

@@ -217,6 +217,18 @@ int fdtd_ctx_autotune(fdtd_ctx *ctx, int trial_steps, hipStream_t stream, int *t
 
 /* The launch shape chosen for the context's fused kernel. */
 int fdtd_ctx_launch_shape(fdtd_ctx *ctx, int *tile_x, int *tile_y, int *kchunk, int *blocks);
+/* The launch shape fdtd_ctx_create would take for this grid and extension
+ * (no counterpart in the reference, whose launch geometry is fixed at
+ * (ceil(nx/128), ny, nz) x 128 threads, fdtd.cu:129-490): 4 tile rows; for
+ * 16-bit storage two cells per lane (tile_x 128) where 128-cell rows are at
+ * least 90% full and a pair launch fills the GPU; the z-chunk of least cost
+ * among 4, 8 and 16 planes, a launch of B blocks on R resident ones costing
+ * ceil(B/R) rounds of kchunk + 2 planes.  ext's tile_y, kchunk, lane_cells
+ * and tile_x fix their parts.  cus: compute units of the target GPU (0 = the
+ * current device's, or 256 when none can be queried).  Allocates nothing and
+ * launches nothing. */
+int fdtd_default_launch_shape(const FdtdParams *p, const FdtdExt *ext, int cus, int *tile_x, int *tile_y,
+                              int *kchunk, int *lane_cells);
 
 /* Times every `every`-th fused-kernel launch (the first, then each `every`
  * launches; every >= 1), up to `capacity` of them (capacity 0 disables):

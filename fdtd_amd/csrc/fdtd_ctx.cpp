@@ -778,7 +778,10 @@ extern "C" int fdtd_ctx_halo_unpack(fdtd_ctx *c, int dir, const void *src, hipSt
         pc.src[q] = (const uint4 *)((const char *)src + q * pb);
         pc.dst[q] = (uint4 *)(fields[q] + (size_t)k * pb);
     }
-    c->fields_set = true;
+    // Zero-term rows stay on: the message holds cells a neighbour's fused
+    // kernel stepped (fdtd_ctx_halo_pack), so a non-finite E^{n-1} or D^{n-1}
+    // in the ghost plane still implies a non-finite E^n or D^n there, and the
+    // ghost plane's own results are overwritten by the next exchange.
     const hipError_t e = launch_plane_copy(pc, stream);
     return e == hipSuccess ? 0 : fdtd_hip_fail(e, "halo unpack");
 }

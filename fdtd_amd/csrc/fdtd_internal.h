@@ -110,11 +110,19 @@ inline unsigned mur_skip_mask(int mur_x)
 // the 256-byte row segment a wave64 of fp32 lanes reads (or a 128-byte one
 // of 16-bit lanes) never straddles an extra line.  Measured on MI355X: a
 // 100x100x130 grid streams at 2.5 TB/s with rows packed (pitch = nx) and a
-// 128^3 one at 4.4 TB/s.
+// 128^3 one at 4.4 TB/s.  Rows of 4 KiB or more whose length is a multiple
+// of 2 KiB get one more line: such strides map the rows of a tile, and the
+// planes of a z-march, onto the same DRAM channels.  Fused kernel, box
+// phantom, best z-chunk each way (Gcell-updates/s): 2048x512x300 55.9 ->
+// 63.1, 1536x1024x200 55.6 -> 62.7, 2048x1024x150 52.1 -> 60.8,
+// 1000x1000x300 (pitch 1024) 56.4 -> 59.9, 1024x1024x200 60.2 -> 61.7; 2 KiB
+// rows (512 fp32) gain nothing (512^3 61.5 -> 61.7), 1 KiB rows lose.
 inline int fused_pitch(int nx, size_t esz)
 {
     const int q = (int)(128 / esz);
-    return (nx + q - 1) / q * q;
+    const int p = (nx + q - 1) / q * q;
+    const size_t bytes = (size_t)p * esz;
+    return bytes >= 4096 && bytes % 2048 == 0 ? p + q : p;
 }
 
 // Guard (in elements) the fused kernel needs around every field array and

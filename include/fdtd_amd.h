@@ -154,6 +154,9 @@ int fdtd_ctx_layout(fdtd_ctx *ctx, int *pitch, long long *plane, int *elem_bytes
 #define FDTD_HALO_DOWN 1
 int fdtd_ctx_halo_bytes(fdtd_ctx *ctx, int dir, size_t *bytes);
 int fdtd_ctx_halo_pack(fdtd_ctx *ctx, int dir, void *dst, hipStream_t stream);
+/* src must hold a neighbour's fdtd_ctx_halo_pack message of the same step (the
+ * zero-term rows of load_zero_terms = 0 rely on its cells having been stepped
+ * by a fused kernel). */
 int fdtd_ctx_halo_unpack(fdtd_ctx *ctx, int dir, const void *src, hipStream_t stream);
 /* Storage scales in use (1 for FDTD_STORE_F32).  Auto scales, fixed at the
  * first step, come from the source waveform: E and D/eps0 take a third of

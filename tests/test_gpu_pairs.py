@@ -40,7 +40,8 @@ def run(shape, storage, lane_cells, kchunk, steps, seed):
 
 @pytest.mark.parametrize("storage", [F.STORE_F16, F.STORE_BF16])
 @pytest.mark.parametrize("shape,kchunk", [((37, 29, 23), 0), ((130, 17, 9), 4), ((129, 12, 14), 8), ((128, 9, 10), 3),
-                                          ((257, 6, 7), 0), ((64, 33, 40), 16), ((5, 6, 7), 0), ((200, 20, 30), 5)])
+                                          ((257, 6, 7), 0), ((64, 33, 40), 16), ((5, 6, 7), 0), ((200, 20, 30), 5),
+                                          ((2048, 6, 7), 0)])   # 4 KiB rows, padded by a line
 def test_pairs_equal_single_cells(storage, shape, kchunk):
     print(f"\n[pairs] {shape} kc={kchunk} storage={storage}", flush=True)
     a, pa, sa = run(shape, storage, 2, kchunk, 25, 71)

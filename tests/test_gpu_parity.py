@@ -143,6 +143,9 @@ FUSED_CASES = [
     ((3, 9, 5), 4, 2, O.DEBYE_REF, F.JZ_STEP, 64),
     ((70, 3, 4), 8, 0, O.DEBYE_FIXED, F.JZ_HOLD3, 64),
     ((66, 7, 3), 2, 1, O.DEBYE_FIXED, F.JZ_STEP, 128),
+    # rows padded by one line past a multiple of 2 KiB (pitch 1056 for both)
+    ((1024, 9, 10), 4, 0, O.DEBYE_FIXED, F.JZ_STEP, 64),
+    ((1000, 7, 9), 2, 5, O.DEBYE_REF, F.JZ_HOLD3, 128),
 ]
 
 
@@ -293,3 +296,14 @@ def test_face_probes_match_fields(shape, kchunk, tile_x):
             want = ctx.get(f)[k, j, i]
             assert want != 0.0 or p >= 12, f"probe {p}: segment face value is zero"
             assert row[p] == np.float64(want), f"step {n + 1} probe {p} {O.FIELDS[f]} ({i},{j},{k})"
+
+
+@pytest.mark.parametrize("storage,nx,pitch", [(F.STORE_F32, 1024, 1056), (F.STORE_F32, 1000, 1056),
+                                              (F.STORE_F32, 1536, 1568), (F.STORE_F32, 512, 512),
+                                              (F.STORE_F32, 256, 256), (F.STORE_F32, 37, 64),
+                                              (F.STORE_F16, 2048, 2112), (F.STORE_F16, 1024, 1024)])
+def test_row_pitch(storage, nx, pitch):
+    """Rows start on 128-byte lines; rows of 4 KiB or more whose length is a
+    multiple of 2 KiB take one more line."""
+    with F.Context(_params((nx, 4, 4)), storage=storage) as ctx:
+        assert ctx.layout()[0] == pitch

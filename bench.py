@@ -105,6 +105,8 @@ def parse():
     ap.add_argument("--load-zero-terms", action="store_true",
                     help="fp32: load E^{n-1} and D^{n-1} on every row, also where the Debye update multiplies "
                          "them by zero (tau = 0); the default skips those loads, bit for bit the same results")
+    ap.add_argument("--time-steps", type=int, default=1, choices=[1, 2],
+                    help="fp32: time steps per fused launch (2: the two-step kernel where it applies)")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape TXxTYxKC (threads per tile row, tile rows, z planes per block), "
                          "e.g. 128x2x6; no autotuning")
@@ -255,7 +257,8 @@ def main():
     if a.shape:
         tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
         fixed = dict(tile_x=tx, tile_y=ty, kchunk=kc, lane_cells=1)
-    ctx = F.Context(w.params, storage=storage, load_zero_terms=int(a.load_zero_terms), **fixed)
+    ctx = F.Context(w.params, storage=storage, load_zero_terms=int(a.load_zero_terms), time_steps=a.time_steps,
+                    **fixed)
     ctx.set_materials(w.ids, w.table)
     ctx.set_sources(w.sources, w.jz)
     ctx.set_probes(w.probes, total)
@@ -282,12 +285,13 @@ def main():
     ctx.run(timing, stream.handle)
     stream.sync()
     kms, launches = ctx.kernel_time()
+    per_launch = max(1, ctx.timed_steps())
     probes = ctx.probes()
     ok = bool(np.isfinite(probes).all()) and bool(np.abs(probes).max() > 0)
 
     value = cells * a.steps / (ms * 1e-3) / 1e9
     kavg_s = kms / launches * 1e-3
-    achieved = bpc * cells / kavg_s / 1e9
+    achieved = bpc * cells * per_launch / kavg_s / 1e9
     shape = ctx.launch_shape()
     launch = f"{shape['tile_x']}x{shape['tile_y']}x{shape['kchunk']}"
     traffic, traffic_src = None, None
@@ -320,9 +324,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "fused_step", "kernel_avg_ms": round(kms / launches, 5),
+                     "kernel": "fused_step" if per_launch == 1 else "fused_step2",
+                     "steps_per_launch": per_launch, "kernel_avg_ms": round(kms / launches, 5),
                      "launches_timed": launches,
-                     "timed_launch_steps": [first + a.steps, first + a.steps + launches],
+                     "timed_launch_steps": [first + a.steps, first + a.steps + launches * per_launch],
                      "bytes_per_cell": bpc},
         "probes_finite_nonzero": ok,
     }

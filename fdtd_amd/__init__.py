@@ -48,7 +48,7 @@ EXPORTS = {
                    "fdtd_ctx_get_field", "fdtd_ctx_field_ptr", "fdtd_step_fused", "fdtd_run",
                    "fdtd_ctx_steps_done", "fdtd_ctx_snapshot", "fdtd_probe_read", "fdtd_probe_peak_units", "fdtd_slices_read",
                    "fdtd_ctx_launch_shape", "fdtd_default_launch_shape",
-                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_scales", "fdtd_ctx_layout",
+                   "fdtd_ctx_set_timing", "fdtd_ctx_kernel_time", "fdtd_ctx_timed_steps", "fdtd_ctx_scales", "fdtd_ctx_layout",
                    "fdtd_ctx_set_energy", "fdtd_energy_read", "fdtd_ctx_save", "fdtd_ctx_load",
                    "fdtd_ctx_autotune", "fdtd_ctx_halo_bytes", "fdtd_ctx_halo_pack", "fdtd_ctx_halo_unpack",
                    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_instance_bytes", "fdtd_batch_add",
@@ -98,7 +98,7 @@ class FdtdExt(C.Structure):
     _fields_ = [("debye_mode", C.c_int), ("jz_index_mode", C.c_int), ("jz_limit", C.c_int),
                 ("tile_y", C.c_int), ("kchunk", C.c_int), ("xcd_remap", C.c_int), ("storage", C.c_int),
                 ("scale_e", C.c_float), ("scale_d", C.c_float), ("scale_h", C.c_float), ("lane_cells", C.c_int),
-                ("tile_x", C.c_int), ("load_zero_terms", C.c_int)]
+                ("tile_x", C.c_int), ("load_zero_terms", C.c_int), ("time_steps", C.c_int)]
 
 
 class FdtdInstance(C.Structure):
@@ -174,6 +174,7 @@ def lib():
         L.fdtd_default_launch_shape.argtypes = [C.POINTER(FdtdParams), C.POINTER(FdtdExt), i, vp, vp, vp, vp]
         L.fdtd_ctx_set_timing.argtypes = [vp, i, i]
         L.fdtd_ctx_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.fdtd_ctx_timed_steps.argtypes = [vp]
         L.fdtd_ctx_scales.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
         L.fdtd_ctx_set_energy.argtypes = [vp, i, i]
         L.fdtd_ctx_save.argtypes = [vp, C.c_char_p]
@@ -275,7 +276,8 @@ class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
     def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0,
-                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0, load_zero_terms=0):
+                 storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0, load_zero_terms=0,
+                 time_steps=0):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))
@@ -286,6 +288,7 @@ class Context:
         ext.lane_cells = lane_cells
         ext.tile_x = tile_x
         ext.load_zero_terms = load_zero_terms
+        ext.time_steps = time_steps
         h = C.c_void_p()
         self._params = params
         check(L.fdtd_ctx_create(C.byref(params), C.byref(ext), C.byref(h)), "fdtd_ctx_create")
@@ -432,6 +435,10 @@ class Context:
         check(lib().fdtd_ctx_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
+    def timed_steps(self):
+        """Steps per timed launch: 1, or 2 for two-step launches (0: none timed)."""
+        return lib().fdtd_ctx_timed_steps(self._h)
+
     def scales(self):
         v = [C.c_double() for _ in range(3)]
         check(lib().fdtd_ctx_scales(self._h, *[C.byref(x) for x in v]), "scales")
@@ -464,7 +471,7 @@ class Context:
 
 
 def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, scales=(0.0, 0.0, 0.0),
-             tile_y=0, kchunk=0, xcd_remap=0, load_zero_terms=0):
+             tile_y=0, kchunk=0, xcd_remap=0, load_zero_terms=0, time_steps=0):
     ext = FdtdExt()
     lib().fdtd_ext_defaults(C.byref(ext))
     ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
@@ -472,6 +479,7 @@ def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, 
     ext.storage = storage
     ext.scale_e, ext.scale_d, ext.scale_h = scales
     ext.load_zero_terms = load_zero_terms
+    ext.time_steps = time_steps
     return ext
 
 

@@ -255,6 +255,7 @@ int fdtd_check_ext(const FdtdExt &e)
     if (e.tile_x != 0 && e.tile_x != 64 && e.tile_x != 128) return fdtd_fail(FDTD_EINVAL, "tile_x %d", e.tile_x);
     if (e.load_zero_terms != 0 && e.load_zero_terms != 1)
         return fdtd_fail(FDTD_EINVAL, "load_zero_terms %d", e.load_zero_terms);
+    if (e.time_steps < 0 || e.time_steps > 2) return fdtd_fail(FDTD_EINVAL, "time_steps %d", e.time_steps);
     return 0;
 }
 

@@ -47,6 +47,12 @@ struct fdtd_ctx {
     unsigned char *ids_base = nullptr;   // allocation holding ids with its guards
     char *E[3][3], *D[3][3], *H[2][3];   // [slot][component], H[buffer][component]
     int hcur = 0;                        // H buffer holding the newest H
+    // Two-step launches: a fourth E and D slot (allocated at the first one),
+    // which takes level n+2 and is then swapped with the ring slot of n-1.
+    char *spare_blob = nullptr;
+    char *Es[3] = {nullptr, nullptr, nullptr}, *Ds[3] = {nullptr, nullptr, nullptr};
+    int kchunk2 = 0;                     // z planes per two-step block
+    bool halo_used = false;              // slab contexts step one step at a time
     unsigned char *ids = nullptr;
     MatCoef *lut = nullptr;
     int nmat = 0;
@@ -79,6 +85,7 @@ struct fdtd_ctx {
     std::vector<hipEvent_t> tev;  // start/end pairs around fused launches
     int tcap = 0, tused = 0;
     int tevery = 1, tseen = 0;    // time launches tseen % tevery == 0
+    int tkind = 0;                // steps per timed launch (1 or 2; 0 before the first): one kind is timed
 
     int n = 0;  // steps done
 };
@@ -150,6 +157,7 @@ void set_shape(fdtd_ctx *c, int ty, int kchunk, int cpl, int txw = 64)
     c->tiles_x = (c->nx + width - 1) / width;
     c->tiles_y = (c->ny + ty - 1) / ty;
     c->tiles_z = (c->nz + c->kchunk - 1) / c->kchunk;
+    c->kchunk2 = c->kchunk;
 }
 
 // Launch shape a context takes for a grid, from launch-shape sweeps on
@@ -198,6 +206,7 @@ extern "C" int fdtd_ext_defaults(FdtdExt *ext)
     ext->lane_cells = 0;
     ext->tile_x = 0;
     ext->load_zero_terms = 0;
+    ext->time_steps = 0;
     return 0;
 }
 
@@ -300,6 +309,7 @@ extern "C" int fdtd_ctx_destroy(fdtd_ctx *c)
     if (!c) return 0;
     if (c->side) (void)hipStreamSynchronize(c->side);
     (void)hipFree(c->blob);
+    (void)hipFree(c->spare_blob);
     (void)hipFree(c->ids_base);
     (void)hipFree(c->lut);
     for (int r = 0; r < 6; r++) (void)hipFree(c->rp[r]);
@@ -469,17 +479,14 @@ extern "C" int fdtd_ctx_field_ptr(fdtd_ctx *c, int field, int level, void **dev)
     return 0;
 }
 
-extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
+namespace {
+
+// Kernel arguments of step n -> n+1 with the context's one-step launch shape:
+// ring slot n % 3 written, (n + 2) % 3 holds level n, (n + 1) % 3 level n-1.
+void fill_step_args(fdtd_ctx *c, int n, StepArgs &a)
 {
-    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
-    resolve_scales(c);
-    int rc = upload_lut(c);
-    if (rc) return rc;
-    const int n = c->n;
     const int t = n % 3, s1 = (n + 2) % 3, s2 = (n + 1) % 3;
     const int ho = c->hcur, hn = c->hcur ^ 1;
-
-    StepArgs a;
     a.hxo = c->H[ho][0]; a.hyo = c->H[ho][1]; a.hzo = c->H[ho][2];
     a.hxn = c->H[hn][0]; a.hyn = c->H[hn][1]; a.hzn = c->H[hn][2];
     a.ex0 = c->E[s1][0]; a.ey0 = c->E[s1][1]; a.ez0 = c->E[s1][2];
@@ -515,30 +522,29 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     // zero-term rows rely on E^{n-1}, D^{n-1} being finite wherever E^n, D^n are, which stepping from
     // zero (or from any state the steps made) keeps; a state set from outside may not
     a.zskip = (c->storage == FDTD_STORE_F32 && c->ext.load_zero_terms == 0 && !c->fields_set) ? 1 : 0;
-    const bool timed = c->tused < c->tcap && c->tseen++ % c->tevery == 0;
-    hipError_t e = timed ? launch_fused_step(a, c->ty, c->storage, stream, c->tev[2 * c->tused], c->tev[2 * c->tused + 1])
-                         : launch_fused_step(a, c->ty, c->storage, stream);
-    if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step");
-    if (timed) c->tused++;
+}
 
+MurArgs mur_args(const fdtd_ctx *c, void *const *target, void *const *source)
+{
     MurArgs m;
-    m.exT = a.exT; m.eyT = a.eyT; m.ezT = a.ezT;
-    m.exS = a.ex0; m.eyS = a.ey0; m.ezS = a.ez0;
+    m.exT = target[0]; m.eyT = target[1]; m.ezT = target[2];
+    m.exS = source[0]; m.eyS = source[1]; m.ezS = source[2];
     m.rpx0 = c->rp[0]; m.rpy0 = c->rp[1]; m.rpz0 = c->rp[2];
     m.rpxEnd = c->rp[3]; m.rpyEnd = c->rp[4]; m.rpzEnd = c->rp[5];
     m.nx = c->nx; m.ny = c->ny; m.nz = c->nz; m.pitch = c->pitch; m.plane = c->plane;
     m.dt = c->dt; m.hx = c->dx; m.hy = c->dy; m.hz = c->dz; m.mu0 = c->mu0; m.eps0 = c->eps0;
-    mur_segments(c->nx, c->ny, c->nz, m.seg_start, mur_skip_mask(a.mur_x));
+    return m;
+}
 
-    c->n = n + 1;
-    c->hcur = hn;
-
-    // The step's probe row is recorded by the Mur launch (newest levels).
+// The probe row of the step just done, reading the fields given (newest
+// levels); p.nprobe = 0 when nothing is recorded.
+ProbeArgs probe_row(fdtd_ctx *c, char *const *fields)
+{
     ProbeArgs p;
     p.nprobe = 0;
     if (c->nprobe > 0 && c->probe_rows < c->probe_cap) {
         for (int f = 0; f < 9; f++) {
-            p.field[f] = field_dev(c, f, 0);
+            p.field[f] = fields[f];
             p.factor[f] = field_factor(c, f);
         }
         p.probes = c->probes;
@@ -548,8 +554,12 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
         p.out = c->probe_out;
         p.row = c->probe_rows++;
     }
-    e = launch_mur_probes(m, p, c->storage, stream);
-    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces/probes");
+    return p;
+}
+
+// Energy sum and z-slice capture of step c->n, when due.
+int after_step(fdtd_ctx *c, hipStream_t stream)
+{
     if (c->energy_interval > 0 && c->n % c->energy_interval == 0 && c->energy_rows < c->energy_cap) {
         EnergyArgs g;
         for (int q = 0; q < 3; q++) {
@@ -562,7 +572,7 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
         g.hh_factor = (double)c->mu0 * field_factor(c, FDTD_HX) * field_factor(c, FDTD_HX);
         g.partial = c->energy_out + c->energy_cap;
         g.out = c->energy_out + c->energy_rows;
-        e = launch_energy(g, c->storage, stream);
+        const hipError_t e = launch_energy(g, c->storage, stream);
         if (e != hipSuccess) return fdtd_hip_fail(e, "energy");
         c->energy_steps.push_back(c->n);
         c->energy_rows++;
@@ -583,12 +593,152 @@ extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
     return 0;
 }
 
+// Whether to time this launch of `kind` steps: every tevery-th launch of the
+// kind timed first, up to the capacity.
+bool timed_launch(fdtd_ctx *c, int kind)
+{
+    if (c->tused >= c->tcap || (c->tkind != 0 && c->tkind != kind)) return false;
+    if (c->tseen++ % c->tevery != 0) return false;
+    c->tkind = kind;
+    return true;
+}
+
+// Whether steps n+1 and n+2 may run as one two-step launch: fp32 storage
+// stepped from zero (the spare slot's never-written cells then hold what
+// the ring slot it replaces holds: zero), no halo exchange between the two,
+// and nothing that needs H^{n+1/2} or all fields of step n+1 (H probes, an
+// energy sum or a slice at n+1).
+bool two_step_ok(const fdtd_ctx *c)
+{
+    if (c->ext.time_steps != 2 || c->storage != FDTD_STORE_F32 || c->fields_set || c->halo_used) return false;
+    if (c->nprobe > 0)
+        for (int f : c->probe_field)
+            if (f >= FDTD_HX && f <= FDTD_HZ) return false;
+    const int m = c->n + 1;
+    if (c->energy_interval > 0 && m % c->energy_interval == 0 && c->energy_rows < c->energy_cap) return false;
+    if (c->slice_interval > 0 && m % c->slice_interval == 0 && c->slice_count < c->slice_cap) return false;
+    return true;
+}
+
+int alloc_spare(fdtd_ctx *c)
+{
+    if (c->spare_blob) return 0;
+    const size_t stride = (c->Np + kArraySkew) * c->esz;
+    const size_t bytes = 6 * stride + 2 * (size_t)c->guard * c->esz;
+    HIP_TRY(hipMalloc(&c->spare_blob, bytes));
+    HIP_TRY(hipMemset(c->spare_blob, 0, bytes));
+    char *q = c->spare_blob + c->guard * c->esz;
+    for (int k = 0; k < 3; k++) { c->Es[k] = q; q += stride; }
+    for (int k = 0; k < 3; k++) { c->Ds[k] = q; q += stride; }
+    return 0;
+}
+
+// Steps n -> n+1 -> n+2 in one fused launch (fused2_tile), then the probe
+// row of step n+1 (E and D of level n+1 as stored), step n+2's Mur faces
+// and its probe row.
+int step_two(fdtd_ctx *c, hipStream_t stream)
+{
+    resolve_scales(c);
+    if (int rc = upload_lut(c)) return rc;
+    if (int rc = alloc_spare(c)) return rc;
+    const int n = c->n;
+    Step2Args s;
+    fill_step_args(c, n, s.A);
+    const int rows = kTwoStepRows;
+    s.A.kchunk = c->kchunk2;
+    two_step_tiles(c->nx, c->ny, c->nz, c->kchunk2, rows, &s.A.tiles_x, &s.A.tiles_y, &s.A.tiles_z);
+    s.exS = c->Es[0]; s.eyS = c->Es[1]; s.ezS = c->Es[2];
+    s.dxS = c->Ds[0]; s.dyS = c->Ds[1]; s.dzS = c->Ds[2];
+    const float jz2 = fdtd_jz_at(c->jz.data(), (int)c->jz.size(), c->ext.jz_limit, c->ext.jz_index_mode, n + 1);
+    s.jz2 = fdtd_jz_stored(jz2, c->storage, c->eps0, c->sD);
+    s.rpy0 = c->rp[1]; s.rpz0 = c->rp[2]; s.rpyEnd = c->rp[4]; s.rpzEnd = c->rp[5];
+    s.dt = c->dt; s.hy = c->dy; s.hz = c->dz; s.mu0 = c->mu0; s.eps0 = c->eps0;
+    const bool timed = timed_launch(c, 2);
+    hipError_t e = timed ? launch_fused_step2(s, rows, stream, c->tev[2 * c->tused], c->tev[2 * c->tused + 1])
+                         : launch_fused_step2(s, rows, stream);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step2");
+    if (timed) c->tused++;
+
+    // step n+1: level n+1 is in ring slot t; no H probes (two_step_ok)
+    const int t = n % 3, tb = (n + 1) % 3;
+    c->n = n + 1;
+    {
+        char *fields[9];
+        for (int q = 0; q < 3; q++) {
+            fields[q] = c->D[t][q];
+            fields[3 + q] = c->H[c->hcur][q];
+            fields[6 + q] = c->E[t][q];
+        }
+        ProbeArgs p = probe_row(c, fields);
+        if (p.nprobe > 0) {
+            e = launch_probes(p, c->storage, stream);
+            if (e != hipSuccess) return fdtd_hip_fail(e, "probes");
+        }
+    }
+    // step n+2: the spare takes ring slot (n+1) % 3 (level n-1 before)
+    for (int q = 0; q < 3; q++) {
+        std::swap(c->E[tb][q], c->Es[q]);
+        std::swap(c->D[tb][q], c->Ds[q]);
+    }
+    c->n = n + 2;
+    c->hcur ^= 1;
+    void *tgt[3] = {c->E[tb][0], c->E[tb][1], c->E[tb][2]};
+    void *src[3] = {c->E[t][0], c->E[t][1], c->E[t][2]};
+    MurArgs m = mur_args(c, tgt, src);
+    mur_segments(c->nx, c->ny, c->nz, m.seg_start, 0);
+    char *fields[9];
+    for (int f = 0; f < 9; f++) fields[f] = field_dev(c, f, 0);
+    ProbeArgs p = probe_row(c, fields);
+    e = launch_mur_probes(m, p, c->storage, stream);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces/probes");
+    return after_step(c, stream);
+}
+
+}  // namespace
+
+extern "C" int fdtd_step_fused(fdtd_ctx *c, hipStream_t stream)
+{
+    if (!c) return fdtd_fail(FDTD_EINVAL, "null context");
+    resolve_scales(c);
+    int rc = upload_lut(c);
+    if (rc) return rc;
+    const int n = c->n;
+    StepArgs a;
+    fill_step_args(c, n, a);
+    const bool timed = timed_launch(c, 1);
+    hipError_t e = timed ? launch_fused_step(a, c->ty, c->storage, stream, c->tev[2 * c->tused], c->tev[2 * c->tused + 1])
+                         : launch_fused_step(a, c->ty, c->storage, stream);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "fused_step");
+    if (timed) c->tused++;
+
+    void *tgt[3] = {a.exT, a.eyT, a.ezT};
+    void *src[3] = {(void *)a.ex0, (void *)a.ey0, (void *)a.ez0};
+    MurArgs m = mur_args(c, tgt, src);
+    mur_segments(c->nx, c->ny, c->nz, m.seg_start, mur_skip_mask(a.mur_x));
+
+    c->n = n + 1;
+    c->hcur ^= 1;
+
+    // The step's probe row is recorded by the Mur launch (newest levels).
+    char *fields[9];
+    for (int f = 0; f < 9; f++) fields[f] = field_dev(c, f, 0);
+    ProbeArgs p = probe_row(c, fields);
+    e = launch_mur_probes(m, p, c->storage, stream);
+    if (e != hipSuccess) return fdtd_hip_fail(e, "mur_faces/probes");
+    return after_step(c, stream);
+}
+
 extern "C" int fdtd_run(fdtd_ctx *c, int steps, hipStream_t stream)
 {
     if (!c || steps < 0) return fdtd_fail(FDTD_EINVAL, "bad run");
-    for (int s = 0; s < steps; s++) {
-        int rc = fdtd_step_fused(c, stream);
-        if (rc) return rc;
+    for (int s = 0; s < steps;) {
+        if (steps - s >= 2 && two_step_ok(c)) {
+            if (int rc = step_two(c, stream)) return rc;
+            s += 2;
+        } else {
+            if (int rc = fdtd_step_fused(c, stream)) return rc;
+            s += 1;
+        }
     }
     return 0;
 }
@@ -675,6 +825,7 @@ extern "C" int fdtd_ctx_set_timing(fdtd_ctx *c, int capacity, int every)
     c->tused = 0;
     c->tevery = every;
     c->tseen = 0;
+    c->tkind = 0;
     c->tev.assign(2 * (size_t)capacity, nullptr);
     for (auto &ev : c->tev) HIP_TRY(hipEventCreate(&ev));
     c->tcap = capacity;
@@ -695,6 +846,8 @@ extern "C" int fdtd_ctx_kernel_time(fdtd_ctx *c, double *ms_total, int *launches
     if (launches) *launches = c->tused;
     return 0;
 }
+
+extern "C" int fdtd_ctx_timed_steps(fdtd_ctx *c) { return c ? c->tkind : FDTD_EINVAL; }
 
 extern "C" int fdtd_ctx_scales(fdtd_ctx *c, double *scale_e, double *scale_d, double *scale_h)
 {
@@ -756,6 +909,7 @@ extern "C" int fdtd_ctx_halo_pack(fdtd_ctx *c, int dir, void *dst, hipStream_t s
     char *fields[kHaloFields];
     long long k;
     if (int rc = halo_planes(c, dir, true, pc, fields, &k)) return rc;
+    c->halo_used = true;
     const size_t pb = (size_t)pc.words * 16;
     for (int q = 0; q < pc.n; q++) {
         pc.src[q] = (const uint4 *)(fields[q] + (size_t)k * pb);
@@ -773,6 +927,7 @@ extern "C" int fdtd_ctx_halo_unpack(fdtd_ctx *c, int dir, const void *src, hipSt
     char *fields[kHaloFields];
     long long k;
     if (int rc = halo_planes(c, dir, false, pc, fields, &k)) return rc;
+    c->halo_used = true;
     const size_t pb = (size_t)pc.words * 16;
     for (int q = 0; q < pc.n; q++) {
         pc.src[q] = (const uint4 *)((const char *)src + q * pb);
@@ -958,6 +1113,8 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     // back to the zero state at step 0
     e = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = hipMemset(c->blob, 0, (24 * (c->Np + kArraySkew) + 2 * (size_t)c->guard) * c->esz);
+    if (e == hipSuccess && c->spare_blob)
+        e = hipMemset(c->spare_blob, 0, (6 * (c->Np + kArraySkew) + 2 * (size_t)c->guard) * c->esz);
     if (!rc && e != hipSuccess) rc = fdtd_hip_fail(e, "autotune reset");
     c->n = 0;
     c->hcur = 0;
@@ -967,6 +1124,7 @@ extern "C" int fdtd_ctx_autotune(fdtd_ctx *c, int trial_steps, hipStream_t strea
     c->tcap = tcap;
     c->tused = 0;
     c->tseen = 0;
+    c->tkind = 0;
     set_shape(c, bty, bkc, bcpl, btxw);
     if (rc) return rc;
     if (tile_y) *tile_y = bty;

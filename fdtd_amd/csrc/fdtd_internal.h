@@ -183,6 +183,33 @@ int stream_mix_resident_blocks();
 // t0, t1 (optional): events that take the launch's start and end times.
 hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t s, hipEvent_t t0 = nullptr,
                              hipEvent_t t1 = nullptr);
+
+// Two time steps in one launch (fp32 storage, the exact form).  A holds the
+// roles of step n -> n+1 (level n+1 goes to A.exT.., A.dxT..); step n+1 ->
+// n+2 writes level n+2 into the spare slot exS.., dxS.. and H^{n+3/2} into
+// A.hxn..  A.tiles_* and A.kchunk describe the two-step tile grid
+// (two_step_tiles); A.mur_x is not read (all six faces of step n+1 are
+// applied in the kernel, those of step n+2 are left to the Mur kernel).
+struct Step2Args {
+    StepArgs A;
+    void *exS, *eyS, *ezS;                   // E^{n+2}
+    void *dxS, *dyS, *dzS;                   // D^{n+2}
+    float jz2;                               // the source value of step n+1 -> n+2
+    const float *rpy0, *rpyEnd, *rpz0, *rpzEnd;   // Mur planes of the y and z faces (MurArgs layout)
+    float dt, hy, hz, mu0, eps0;
+};
+// Rows of the two-step kernel's 64-lane tiles; each tile stores the inner
+// 62 x (rows - 2) cells.
+constexpr int kTwoStepRows = 8;
+constexpr int kTwoStepCols = 62;
+inline void two_step_tiles(int nx, int ny, int nz, int kchunk, int rows, int *tx, int *ty, int *tz)
+{
+    *tx = (nx + kTwoStepCols - 1) / kTwoStepCols;
+    *ty = (ny + rows - 3) / (rows - 2);
+    *tz = (nz + kchunk - 1) / kchunk;
+}
+hipError_t launch_fused_step2(const Step2Args &a, int rows, hipStream_t s, hipEvent_t t0 = nullptr,
+                              hipEvent_t t1 = nullptr);
 hipError_t launch_mur(const MurArgs &a, int storage, hipStream_t s);
 hipError_t launch_probes(const ProbeArgs &a, int storage, hipStream_t s);
 // Halo planes of a z-slab: n runs of `words` 16-byte words each.

@@ -919,6 +919,414 @@ __global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) vo
     fused_tile<TY, T, EXACT, TXW, ZS>(A, bx, by, bz);
 }
 
+// Two time steps per launch (fp32 storage, the reference's arithmetic).
+//
+// Step n -> n+1 runs on a 64 x R box of (x, y) columns at origin
+// (62 bx - 1, (R-2) by - 1): one wave per box row, one cell per lane, with
+// the E^n staging, halo H and z-march of fused_tile.  Step n+1 -> n+2 needs
+// E^{n+1} one cell around each of its cells, so a tile stores only the
+// inner 62 x (R-2) cells of its box, and the boxes of neighbouring tiles
+// overlap by two columns and two rows.  Both steps march z together, the
+// second one plane behind the first: iteration kA computes step n -> n+1
+// on plane kA and step n+1 -> n+2 on plane kA-1.  Between the two, the first
+// step's six Mur faces (fdtd_calculations.cu:216-428) are applied inside the
+// box, so the second step reads E^{n+1} as the reference's step leaves it;
+// the second step's faces are left to the Mur kernel.  H^{n+1/2} stays on
+// the chip.  Step n -> n+1 reads E^{n-1} and D^{n-1} on every box cell,
+// including the ring that neighbouring tiles store, so step n+1 -> n+2
+// writes level n+2 into a spare slot rather than over level n-1.
+// Per cell and launch: 61 bytes read (H^{n-1/2}, E^n, E^{n-1}, D^n, D^{n-1},
+// the id) and 60 written (H^{n+3/2}, E and D of levels n+1 and n+2).
+template <int R>
+__device__ __forceinline__ void fused2_tile(const Step2Args &S, int bx, int by, int bz)
+{
+    // The +y halo row (j0 + R <= ny + R - 2) and plane nz stay inside the
+    // arrays' guard of one plane and 10 rows.
+    static_assert(R >= 4 && R <= 12, "tile rows");
+    constexpr int W = 64;
+    constexpr uint32_t ES = sizeof(float);
+    using X = Elem<float>;
+    const StepArgs &A = S.A;
+    __shared__ float sE[2][3][R + 2][W + 2];   // E^n of planes kA and kA+1, one-cell halo
+    __shared__ float sH[3][R + 1][W + 1];      // H^{n+1/2} of plane kA with its -x/-y halo
+    __shared__ float sF[2][3][R + 1][W + 1];   // E^{n+1} of planes kA-1 and kA (box; padding row and column)
+    __shared__ float sG[3][R + 1][W + 1];      // H^{n+3/2} of plane kA-1 (box at [1..R][1..W])
+    __shared__ float sJunk[2][W];
+
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int i0 = bx * (W - 2) - 1, j0 = by * (R - 2) - 1;
+    const int k0 = bz * A.kchunk;
+    const int k1 = min(k0 + A.kchunk, nz);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool in = i >= 0 && i < nx && j >= 0 && j < ny;
+    const bool own = in && tx >= 1 && tx <= W - 2 && ty >= 1 && ty <= R - 2;   // the cells this tile stores
+
+    const int nmat = A.nmat;
+    const double *sLut = lut_lds();
+    const double *lG1 = sLut, *lG2 = sLut + nmat, *lG3 = sLut + 2 * nmat;
+    const float *lA = (const float *)(sLut + 3 * nmat), *lC = lA + nmat, *lD = lA + 2 * nmat;
+
+    const uint32_t gb = A.guard * ES;
+    auto sh = [&](const void *p) { return (const float *)((const char *)p - gb); };
+    auto shw = [&](void *p) { return (float *)((char *)p - gb); };
+    const float *hxo = sh(A.hxo), *hyo = sh(A.hyo), *hzo = sh(A.hzo);
+    const float *ex0 = sh(A.ex0), *ey0 = sh(A.ey0), *ez0 = sh(A.ez0);
+    const float *ex1 = sh(A.ex1), *ey1 = sh(A.ey1), *ez1 = sh(A.ez1);
+    const float *dx0 = sh(A.dx0), *dy0 = sh(A.dy0), *dz0 = sh(A.dz0);
+    const float *dx1 = sh(A.dx1), *dy1 = sh(A.dy1), *dz1 = sh(A.dz1);
+    float *hxn = shw(A.hxn), *hyn = shw(A.hyn), *hzn = shw(A.hzn);
+    float *exT = shw(A.exT), *eyT = shw(A.eyT), *ezT = shw(A.ezT);
+    float *dxT = shw(A.dxT), *dyT = shw(A.dyT), *dzT = shw(A.dzT);
+    float *exS = shw(S.exS), *eyS = shw(S.eyS), *ezS = shw(S.ezS);
+    float *dxS = shw(S.dxS), *dyS = shw(S.dyS), *dzS = shw(S.dzS);
+    const uint8_t *ids = A.ids - A.guard;
+
+    const uint32_t syb = (uint32_t)A.pitch * ES, szb = (uint32_t)A.plane * ES;
+    // 32-bit offsets wrap for the box's -1 row and column (and the -2 halo).
+    const int il = min(i, A.pitch - 1);
+    const uint32_t c_own = gb + (uint32_t)j * syb + (uint32_t)il * ES;
+    const uint32_t c_rlo = gb + (uint32_t)(j0 - 1) * syb + (uint32_t)il * ES;
+    const uint32_t c_rhi = gb + (uint32_t)(j0 + R) * syb + (uint32_t)il * ES;
+    const int side = tx >> 5;
+    const int cr = min(tx & 31, R + 1);
+    const int cl = side ? W + 1 : 0;
+    const int ci = side ? i0 + W : i0 - 1;
+    const uint32_t c_col = gb + (uint32_t)(j0 - 1 + cr) * syb + (uint32_t)ci * ES;
+    const bool w_rlo = (ty == 0), w_rhi = (ty == R - 1), w_col = (ty == 1);
+
+    const float chx = A.chx, chy = A.chy, chz = A.chz;
+    const float cdx = A.cdx, cdy = A.cdy, cdz = A.cdz;
+
+    float pe[3], pr[3], pq[2], pc[3];
+    auto fetch_e = [&](int k) {
+        const uint32_t kb = (uint32_t)k * szb;
+        pe[0] = X::ld(ex0, kb + c_own); pe[1] = X::ld(ey0, kb + c_own); pe[2] = X::ld(ez0, kb + c_own);
+        if (w_rlo) { pr[0] = X::ld(ex0, kb + c_rlo); pr[1] = X::ld(ey0, kb + c_rlo); pr[2] = X::ld(ez0, kb + c_rlo); }
+        if (w_rhi) { pq[0] = X::ld(ex0, kb + c_rhi); pq[1] = X::ld(ez0, kb + c_rhi); }
+        if (w_col) { pc[0] = X::ld(ex0, kb + c_col); pc[1] = X::ld(ey0, kb + c_col); pc[2] = X::ld(ez0, kb + c_col); }
+    };
+    auto stage_e = [&](int buf) {
+        sE[buf][0][ty + 1][tx + 1] = pe[0]; sE[buf][1][ty + 1][tx + 1] = pe[1]; sE[buf][2][ty + 1][tx + 1] = pe[2];
+        if (w_rlo) { sE[buf][0][0][tx + 1] = pr[0]; sE[buf][1][0][tx + 1] = pr[1]; sE[buf][2][0][tx + 1] = pr[2]; }
+        if (w_rhi) { sE[buf][0][R + 1][tx + 1] = pq[0]; sE[buf][2][R + 1][tx + 1] = pq[1]; }
+        if (w_col) { sE[buf][0][cr][cl] = pc[0]; sE[buf][1][cr][cl] = pc[1]; sE[buf][2][cr][cl] = pc[2]; }
+    };
+    struct Own {
+        float hxo, hyo, hzo, dxn, dyn, dzn, dxm, dym, dzm, exm, eym, ezm, hh0, hh1;
+        uint32_t id;
+    };
+    auto fetch_own = [&](int k, Own &o) {
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        o.hxo = X::ld(hxo, ob); o.hyo = X::ld(hyo, ob); o.hzo = X::ld(hzo, ob);
+        o.dxn = X::ld_nt(dx0, ob); o.dyn = X::ld_nt(dy0, ob); o.dzn = X::ld_nt(dz0, ob);
+        o.dxm = X::ld_nt(dx1, ob); o.dym = X::ld_nt(dy1, ob); o.dzm = X::ld_nt(dz1, ob);
+        o.exm = X::ld_nt(ex1, ob); o.eym = X::ld_nt(ey1, ob); o.ezm = X::ld_nt(ez1, ob);
+        o.id = __builtin_nontemporal_load(ids + ob / ES);
+        o.hh0 = 0.f; o.hh1 = 0.f;
+        if (w_rlo) { const uint32_t oh = (uint32_t)k * szb + c_rlo; o.hh0 = X::ld(hxo, oh); o.hh1 = X::ld(hzo, oh); }
+        if (w_col) { const uint32_t oh = (uint32_t)k * szb + c_col; o.hh0 = X::ld(hyo, oh); o.hh1 = X::ld(hzo, oh); }
+    };
+
+    // Step n -> n+1 starts one plane below the chunk (the second step's
+    // D^{n+2} at k0 needs H^{n+3/2}, hence E^{n+1}, at k0-1) and ends one
+    // above it (H^{n+3/2} at k1-1 needs E^{n+1} at k1).
+    const int kAs = max(k0 - 1, 0);
+    float hxm = 0.0f, hym = 0.0f, hzm = 0.0f;   // H^{n+1/2} of the own cell, plane kA-1
+    int cur = 0;
+    if (kAs > 0) {
+        fetch_e(kAs - 1);
+        stage_e(0);
+        fetch_e(kAs);
+        stage_e(1);
+        __syncthreads();
+        const int k = kAs - 1;
+        const uint32_t ob = (uint32_t)k * szb + c_own;
+        const float hx0 = X::ld(hxo, ob), hy0 = X::ld(hyo, ob);
+        const float exk = sE[0][0][ty + 1][tx + 1], eyk = sE[0][1][ty + 1][tx + 1], ezk = sE[0][2][ty + 1][tx + 1];
+        const float exk1 = sE[1][0][ty + 1][tx + 1], eyk1 = sE[1][1][ty + 1][tx + 1];
+        const float ux = hx0 - chy * (sE[0][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
+        const float uy = hy0 - chz * (exk1 - exk) + chx * (sE[0][2][ty + 1][tx + 2] - ezk);
+        hxm = (i >= 1 && i < nx - 1 && j < ny - 1 && k < nz - 1) ? ux : hx0;
+        hym = (i < nx - 1 && j >= 1 && j < ny - 1 && k < nz - 1) ? uy : hy0;
+        cur = 1;
+        __syncthreads();
+    } else {
+        fetch_e(kAs);
+        stage_e(0);
+    }
+    Own cu, nx_;
+    fetch_own(kAs, cu);
+    fetch_e(kAs + 1);
+
+    const bool hx_ij = (i >= 1 && i < nx - 1 && j < ny - 1);
+    const bool hy_ij = (i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool hz_ij = (i < nx - 1 && j < ny - 1);
+    const bool dx_ij = (i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool dy_ij = (i >= 1 && i < nx - 1 && j < ny - 1);
+    const bool dz_ij = (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1);
+    const bool e_i = (i >= 1);
+    const int jh = j0 - 1;
+    const bool rx_ij = (jh >= 0) && (i >= 1 && i < nx - 1 && jh < ny - 1);
+    const bool rz_ij = (jh >= 0) && (i < nx - 1 && jh < ny - 1);
+    const int ccr = min(max(cr, 1), R);
+    const int jc = j0 - 1 + ccr;
+    const bool col_ok = (side == 0) && (cr >= 1) && (cr <= R) && (i0 >= 1);
+    const bool cy_ij = col_ok && (i0 - 1 < nx - 1) && (jc >= 1 && jc < ny - 1);
+    const bool cz_ij = col_ok && (i0 - 1 < nx - 1) && (jc < ny - 1);
+    float *col_dst_y = col_ok ? &sH[1][ccr][0] : &sJunk[0][tx];
+    float *col_dst_z = col_ok ? &sH[2][ccr][0] : &sJunk[1][tx];
+
+    // Mur faces inside the box: x = 0 at lane 1 of the first column of
+    // tiles, x = nx-1 wherever it falls, y = 0 and ny-1 on whole waves.
+    const bool xface_t = (i0 < 0) || (i0 <= nx - 1 && nx - 1 < i0 + W);
+    const bool yface = (j == 0 || j == ny - 1);
+    const float mdt = S.dt, mhy = S.hy, mhz = S.hz, mmu = S.mu0, mep = S.eps0;
+    const long long nxy = (long long)nx * ny;
+
+    float gxm = 0.0f, gym = 0.0f;              // H^{n+3/2} of the own cell, plane kA-2
+    float pex = 0.f, pey = 0.f, pez = 0.f;     // E^n, own cell, plane kA-1
+    float pdx = 0.f, pdy = 0.f, pdz = 0.f;     // D^n, own cell, plane kA-1
+    float qdx = 0.f, qdy = 0.f, qdz = 0.f;     // D^{n+1} as its slot holds it, own cell, plane kA-1
+    uint32_t pid = 0;                          // id, own cell, plane kA-1
+    const float jzA = A.jz, jzB = S.jz2;
+
+    for (int kA = kAs; kA <= k1; ++kA) {
+        const bool realA = kA < nz;            // plane nz: only the second step's last plane is left
+        const int nxt = cur ^ 1;
+        stage_e(nxt);                          // E^n of plane kA+1
+        const bool more = kA + 1 <= k1 && kA + 1 < nz;
+        if (more) {
+            fetch_own(kA + 1, nx_);
+            fetch_e(kA + 2);
+        }
+        __syncthreads();  // E^n of planes kA and kA+1 staged
+
+        // ---- step n -> n+1, plane kA: H^{n+1/2} of the box and its -x/-y halo
+        const bool kh = (kA < nz - 1), kz = (kA >= 1 && kA < nz - 1);
+        const float exk = sE[cur][0][ty + 1][tx + 1], eyk = sE[cur][1][ty + 1][tx + 1], ezk = sE[cur][2][ty + 1][tx + 1];
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        if (realA) {
+            const float exk1 = sE[nxt][0][ty + 1][tx + 1], eyk1 = sE[nxt][1][ty + 1][tx + 1];
+            const float ux = cu.hxo - chy * (sE[cur][2][ty + 2][tx + 1] - ezk) + chy * (eyk1 - eyk);
+            const float uy = cu.hyo - chz * (exk1 - exk) + chx * (sE[cur][2][ty + 1][tx + 2] - ezk);
+            const float uz = cu.hzo - chx * (sE[cur][1][ty + 1][tx + 2] - eyk) + chy * (sE[cur][0][ty + 2][tx + 1] - exk);
+            hx = (hx_ij && kh) ? ux : cu.hxo;
+            hy = (hy_ij && kh) ? uy : cu.hyo;
+            hz = (hz_ij && kz) ? uz : cu.hzo;
+            sH[0][ty + 1][tx + 1] = hx;
+            sH[1][ty + 1][tx + 1] = hy;
+            sH[2][ty + 1][tx + 1] = hz;
+            if (w_rlo) {  // (i, j0-1): Hx, Hz
+                const float ezh = sE[cur][2][0][tx + 1], eyh = sE[cur][1][0][tx + 1], exh = sE[cur][0][0][tx + 1];
+                const float vx = cu.hh0 - chy * (sE[cur][2][1][tx + 1] - ezh) + chy * (sE[nxt][1][0][tx + 1] - eyh);
+                const float vz = cu.hh1 - chx * (sE[cur][1][0][tx + 2] - eyh) + chy * (sE[cur][0][1][tx + 1] - exh);
+                sH[0][0][tx + 1] = (rx_ij && kh) ? vx : cu.hh0;
+                sH[2][0][tx + 1] = (rz_ij && kz) ? vz : cu.hh1;
+            }
+            if (w_col) {  // (i0-1, j0-1+ccr): Hy, Hz
+                const float exh = sE[cur][0][ccr][0], eyh = sE[cur][1][ccr][0], ezh = sE[cur][2][ccr][0];
+                const float vy = cu.hh0 - chz * (sE[nxt][0][ccr][0] - exh) + chx * (sE[cur][2][ccr][1] - ezh);
+                const float vz = cu.hh1 - chx * (sE[cur][1][ccr][1] - eyh) + chy * (sE[cur][0][ccr + 1][0] - exh);
+                *col_dst_y = (cy_ij && kh) ? vy : cu.hh0;
+                *col_dst_z = (cz_ij && kz) ? vz : cu.hh1;
+            }
+        }
+        __syncthreads();  // H^{n+1/2} of plane kA and its halo staged
+
+        // ---- step n -> n+1, plane kA: D^{n+1} and E^{n+1} of the box, then
+        // the x and z Mur faces (the y faces follow the next barrier)
+        const uint32_t ob = (uint32_t)kA * szb + c_own;
+        float fx = 0.f, fy = 0.f, fz = 0.f;         // E^{n+1}, own cell
+        float dxu = 0.f, dyp = 0.f, dzp = 0.f;      // D^{n+1} as its slot holds it
+        bool rdx = false, rdy = false, rdz = false, stx = false, sty = false, stz = false;
+        if (realA) {
+            rdx = dx_ij && kz;
+            rdy = dy_ij && kz;
+            rdz = (dz_ij && kh) || (cu.id & kSourceBit);
+            const bool reyz = e_i && (kA >= 1);
+            dxu = cu.dxn + cdy * (hz - sH[2][ty][tx + 1]) - cdz * (hy - hym);
+            const float dyu = cu.dyn + cdz * (hx - hxm) - cdx * (hz - sH[2][ty + 1][tx]);
+            float dzu = cu.dzn + cdx * (hy - sH[1][ty + 1][tx]) - cdy * (hx - sH[0][ty][tx + 1]);
+            dzu = (cu.id & kSourceBit) ? dzu - jzA : dzu;
+            dyp = dyu;
+            dzp = dzu;
+            // E cells whose D is not updated read the target slot's value.
+            if (in && reyz && !rdy) dyp = X::ld(dyT, ob);
+            if (in && reyz && !rdz) dzp = X::ld(dzT, ob);
+            const int m = cu.id & kMaterialMask;
+            const float ca = lA[m], cc = lC[m], cd = lD[m];
+            const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
+            fx = e_update(cc, cd, ca, g1, g2, g3, exk, cu.exm, dxu, cu.dxn, cu.dxm);
+            fy = e_update(cc, cd, ca, g1, g2, g3, eyk, cu.eym, dyp, cu.dyn, cu.dym);
+            fz = e_update(cc, cd, ca, g1, g2, g3, ezk, cu.ezm, dzp, cu.dzn, cu.dzm);
+            // Cells outside the E update keep the target slot's value.
+            if (in && !rdx) fx = X::ld(exT, ob);
+            if (in && !reyz) { fy = X::ld(eyT, ob); fz = X::ld(ezT, ob); }
+            stx = rdx;
+            sty = reyz;
+            stz = reyz;
+            // Mur faces read E^{n+1} of their neighbour, a cell no face writes
+            // (so its value before the faces is final), and E^n of both.
+            if (xface_t) {
+                const float eyr = __shfl_down(fy, 1, 64), ezr = __shfl_down(fz, 1, 64), ezl = __shfl_up(fz, 1, 64);
+                if (i == 0 && j >= 0 && j < ny) {
+                    const float4 c = A.murx0[(long long)kA * ny + j];
+                    if (j < ny - 1 && kA >= 1 && kA < nz - 1) {   // Ey, x = 0
+                        fy = c.x * ((c.y * eyr + c.z * sE[cur][1][ty + 1][tx + 2]) - c.w * eyk);
+                        sty = true;
+                    }
+                    if (j >= 1 && j < ny - 1 && kA < nz - 1) {    // Ez, x = 0
+                        fz = c.x * ((c.y * ezr + c.z * sE[cur][2][ty + 1][tx + 2]) - c.w * ezk);
+                        stz = true;
+                    }
+                }
+                if (i == nx - 1 && j >= 0 && j < ny) {
+                    const float4 c = A.murx1[(long long)kA * ny + j];
+                    if (j < ny - 1 && kA >= 1 && kA < nz - 1) {   // Ey, x = nx-1: the source neighbour twice
+                        const float sn = sE[cur][1][ty + 1][tx];
+                        fy = c.x * ((c.y * sn + c.z * sn) - c.w * eyk);
+                        sty = true;
+                    }
+                    if (j >= 1 && j < ny - 1 && kA < nz - 1) {    // Ez, x = nx-1
+                        fz = c.x * ((c.y * ezl + c.z * sE[cur][2][ty + 1][tx]) - c.w * ezk);
+                        stz = true;
+                    }
+                }
+            }
+            if (kA == nz - 1 && in) {   // z = nz-1: neighbour E^{n+1} from the previous plane
+                const long long p = (long long)j * nx + i;
+                const int pb = (kA - 1) & 1;
+                if (i < nx - 1 && j >= 1 && j < ny - 1) {
+                    fx = mur1(mdt, mhz, mmu, mep, S.rpzEnd[nxy + p], S.rpzEnd[p], sF[pb][0][ty][tx], pex, exk);
+                    stx = true;
+                }
+                if (i >= 1 && i < nx - 1 && j < ny - 1) {
+                    fy = mur1(mdt, mhz, mmu, mep, S.rpzEnd[nxy + p], S.rpzEnd[p], sF[pb][1][ty][tx], pey, eyk);
+                    sty = true;
+                }
+            }
+            if (kA == 1 && kAs == 0 && in) {   // z = 0, one plane late: its neighbour is this plane
+                const long long p = (long long)j * nx + i;
+                if (i < nx - 1 && j >= 1 && j < ny - 1) {
+                    const float v = mur1(mdt, mhz, mmu, mep, S.rpz0[p], S.rpz0[nxy + p], fx, exk, pex);
+                    sF[0][0][ty][tx] = v;
+                    if (own && k0 == 0) X::st_nt(exT, c_own, v);
+                }
+                if (i >= 1 && i < nx - 1 && j < ny - 1) {
+                    const float v = mur1(mdt, mhz, mmu, mep, S.rpz0[p], S.rpz0[nxy + p], fy, eyk, pey);
+                    sF[0][1][ty][tx] = v;
+                    if (own && k0 == 0) X::st_nt(eyT, c_own, v);
+                }
+            }
+            const int fb = kA & 1;
+            sF[fb][0][ty][tx] = fx;
+            sF[fb][1][ty][tx] = fy;
+            sF[fb][2][ty][tx] = fz;
+        }
+        __syncthreads();  // E^{n+1} of plane kA staged (before its y faces)
+
+        if (realA && yface) {   // whole waves on y = 0 or ny-1: neighbour row from LDS
+            const int fb = kA & 1;
+            const bool lo = (j == 0);
+            const int jn = lo ? ty + 1 : ty - 1;
+            const float *rp = lo ? S.rpy0 : S.rpyEnd;
+            const long long py = (long long)kA * 2 * nx + i;
+            const long long pf = py + (lo ? 0 : nx), pn = py + (lo ? nx : 0);
+            if (i >= 0 && i < nx - 1 && kA >= 1 && kA < nz - 1) {   // Ex
+                fx = mur1(mdt, mhy, mmu, mep, rp[pf], rp[pn], sF[fb][0][jn][tx], sE[cur][0][jn + 1][tx + 1], exk);
+                sF[fb][0][ty][tx] = fx;
+                stx = true;
+            }
+            if (i >= 1 && i < nx - 1 && kA < nz - 1) {              // Ez
+                fz = mur1(mdt, mhy, mmu, mep, rp[pf], rp[pn], sF[fb][2][jn][tx], sE[cur][2][jn + 1][tx + 1], ezk);
+                sF[fb][2][ty][tx] = fz;
+                stz = true;
+            }
+        }
+        if (realA && own && kA >= k0 && kA < k1) {
+            if (rdx) X::st_nt(dxT, ob, dxu);
+            if (rdy) X::st_nt(dyT, ob, dyp);
+            if (rdz) X::st_nt(dzT, ob, dzp);
+            if (stx) X::st_nt(exT, ob, fx);
+            if (sty) X::st_nt(eyT, ob, fy);
+            if (stz) X::st_nt(ezT, ob, fz);
+        }
+
+        // ---- step n+1 -> n+2, plane kB = kA-1: H^{n+3/2} of the box (its last
+        // row and column are not needed and read padding)
+        const int kB = kA - 1;
+        const int gb1 = kB & 1;
+        const float gxk = sF[gb1][0][ty][tx], gyk = sF[gb1][1][ty][tx], gzk = sF[gb1][2][ty][tx];
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        const bool khB = kB < nz - 1, kzB = kB >= 1 && kB < nz - 1;
+        if (kB >= kAs) {
+            const float vx = hxm - chy * (sF[gb1][2][ty + 1][tx] - gzk) + chy * (fy - gyk);
+            const float vy = hym - chz * (fx - gxk) + chx * (sF[gb1][2][ty][tx + 1] - gzk);
+            const float vz = hzm - chx * (sF[gb1][1][ty][tx + 1] - gyk) + chy * (sF[gb1][0][ty + 1][tx] - gxk);
+            gx = (hx_ij && khB) ? vx : hxm;
+            gy = (hy_ij && khB) ? vy : hym;
+            gz = (hz_ij && kzB) ? vz : hzm;
+            sG[0][ty + 1][tx + 1] = gx;
+            sG[1][ty + 1][tx + 1] = gy;
+            sG[2][ty + 1][tx + 1] = gz;
+        }
+        __syncthreads();  // H^{n+3/2} of plane kB staged
+
+        // ---- step n+1 -> n+2, plane kB: D^{n+2}, E^{n+2} of the stored cells
+        if (kB >= k0 && kB < k1) {
+            const uint32_t obB = (uint32_t)kB * szb + c_own;
+            const bool src = (pid & kSourceBit) != 0;
+            const bool rdx2 = dx_ij && kzB, rdy2 = dy_ij && kzB, rdz2 = (dz_ij && khB) || src;
+            const bool reyz2 = e_i && (kB >= 1);
+            const float dxu2 = qdx + cdy * (gz - sG[2][ty][tx + 1]) - cdz * (gy - gym);
+            const float dyu2 = qdy + cdz * (gx - gxm) - cdx * (gz - sG[2][ty + 1][tx]);
+            float dzu2 = qdz + cdx * (gy - sG[1][ty + 1][tx]) - cdy * (gx - sG[0][ty][tx + 1]);
+            dzu2 = src ? dzu2 - jzB : dzu2;
+            float dyp2 = dyu2, dzp2 = dzu2;
+            if (own && reyz2 && !rdy2) dyp2 = X::ld(dyS, obB);
+            if (own && reyz2 && !rdz2) dzp2 = X::ld(dzS, obB);
+            const int m = pid & kMaterialMask;
+            const float ca = lA[m], cc = lC[m], cd = lD[m];
+            const double g1 = lG1[m], g2 = lG2[m], g3 = lG3[m];
+            const float exn = e_update(cc, cd, ca, g1, g2, g3, gxk, pex, dxu2, qdx, pdx);
+            const float eyn = e_update(cc, cd, ca, g1, g2, g3, gyk, pey, dyp2, qdy, pdy);
+            const float ezn = e_update(cc, cd, ca, g1, g2, g3, gzk, pez, dzp2, qdz, pdz);
+            if (own) {
+                X::st_nt(hxn, obB, gx); X::st_nt(hyn, obB, gy); X::st_nt(hzn, obB, gz);
+                if (rdx2) { X::st_nt(dxS, obB, dxu2); X::st_nt(exS, obB, exn); }
+                if (rdy2) X::st_nt(dyS, obB, dyu2);
+                if (rdz2) X::st_nt(dzS, obB, dzu2);
+                if (reyz2) { X::st_nt(eyS, obB, eyn); X::st_nt(ezS, obB, ezn); }
+            }
+        }
+        if (kB >= kAs) { gxm = gx; gym = gy; }
+        hxm = hx; hym = hy; hzm = hz;
+        pex = exk; pey = eyk; pez = ezk;
+        pdx = cu.dxn; pdy = cu.dyn; pdz = cu.dzn;
+        qdx = dxu; qdy = dyp; qdz = dzp;
+        pid = cu.id;
+        cur = nxt;
+        if (more) cu = nx_;
+    }
+}
+
+// Two steps of one grid per launch: block b runs tile xcd_logical(b) at
+// four waves per SIMD.
+template <int R>
+__global__ __launch_bounds__(64 * R) __attribute__((amdgpu_waves_per_eu(4))) void fused_step2(const Step2Args S)
+{
+    const StepArgs &A = S.A;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int bx = b % A.tiles_x;
+    const int by = (b / A.tiles_x) % A.tiles_y;
+    const int bz = b / (A.tiles_x * A.tiles_y);
+    stage_lut<true>(A.lut, A.nmat, tid, 64 * R);
+    fused2_tile<R>(S, bx, by, bz);
+}
+
 // Batch tables are read through the constant address space: they do not
 // change while a launch runs, so the block-uniform reads become scalar
 // loads and the grid's pointers stay in SGPRs.
@@ -1485,6 +1893,22 @@ hipError_t launch_fused_step(const StepArgs &a, int ty, int storage, hipStream_t
     case kStoreBF16: return launch_fused_t<__hip_bfloat16, false>(a, ty, s, t0, t1);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fused_step2(const Step2Args &a, int rows, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
+{
+    const unsigned nblk = (unsigned)a.A.tiles_x * a.A.tiles_y * a.A.tiles_z;
+    const size_t lds = (size_t)a.A.nmat * (3 * sizeof(double) + 3 * sizeof(float));
+    const void *k = nullptr;
+    switch (rows) {
+    case 6: k = (const void *)fused_step2<6>; break;
+    case 8: k = (const void *)fused_step2<8>; break;
+    case 10: k = (const void *)fused_step2<10>; break;
+    default: return hipErrorInvalidValue;
+    }
+    Step2Args arg = a;
+    void *args[] = {&arg};
+    return hipExtLaunchKernel(k, dim3(nblk), dim3(64, rows), args, lds, s, t0, t1, 0);
 }
 
 namespace {

@@ -71,6 +71,11 @@ typedef struct
                             cells' Debye update multiplies them by exactly zero (tau = 0: vacuum and
                             non-dispersive media), with results unchanged bit for bit; 1 always loads
                             them.  Off after fdtd_ctx_set_field or fdtd_ctx_load. */
+    int time_steps;     /* fp32 storage: time steps per fused launch, 0 = auto (1) or 1, 2.  With 2, fdtd_run
+                           advances pairs of steps in one launch where it can (stepped from zero, no halo
+                           exchange, no H probes, no energy sum or slice on the first step of a pair),
+                           bit for bit as one step at a time; it holds a fourth E and D slot (24 bytes per
+                           cell more). */
 } FdtdExt;
 
 typedef struct fdtd_ctx fdtd_ctx;
@@ -239,9 +244,12 @@ int fdtd_default_launch_shape(const FdtdParams *p, const FdtdExt *ext, int cus, 
  * start and end.  Timing every launch costs the step sequence about 1.5%,
  * so sampling keeps the timed run's wall clock close to an untimed one.
  * fdtd_ctx_kernel_time waits for the recorded pairs and returns their
- * summed duration. */
+ * summed duration.  Only launches of one kind are timed, the kind of the
+ * first timed launch: fdtd_ctx_timed_steps returns its steps per launch (1,
+ * or 2 for two-step launches; 0 before any launch was timed). */
 int fdtd_ctx_set_timing(fdtd_ctx *ctx, int capacity, int every);
 int fdtd_ctx_kernel_time(fdtd_ctx *ctx, double *ms_total, int *launches);
+int fdtd_ctx_timed_steps(fdtd_ctx *ctx);
 
 /* Memory-ceiling calibration (no counterpart in the reference): streams the
  * fused step's access mix without its arithmetic, 15 arrays read and 9

@@ -105,6 +105,8 @@ def parse():
     ap.add_argument("--load-zero-terms", action="store_true",
                     help="fp32: load E^{n-1} and D^{n-1} on every row, also where the Debye update multiplies "
                          "them by zero (tau = 0); the default skips those loads, bit for bit the same results")
+    ap.add_argument("--xcd-remap", type=int, default=None, choices=[0, 1, 2, 3, 4],
+                    help="block -> tile order of the fused kernel (FdtdExt.xcd_remap; default: the library's)")
     ap.add_argument("--time-steps", type=int, default=1, choices=[1, 2],
                     help="fp32: time steps per fused launch (2: the two-step kernel where it applies)")
     ap.add_argument("--shape", default=None,
@@ -258,7 +260,8 @@ def main():
         tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
         fixed = dict(tile_x=tx, tile_y=ty, kchunk=kc, lane_cells=1)
     ctx = F.Context(w.params, storage=storage, load_zero_terms=int(a.load_zero_terms), time_steps=a.time_steps,
-                    **fixed)
+                    xcd_remap=a.xcd_remap, **fixed)
+    remap = a.xcd_remap if a.xcd_remap is not None else F.make_ext().xcd_remap
     ctx.set_materials(w.ids, w.table)
     ctx.set_sources(w.sources, w.jz)
     ctx.set_probes(w.probes, total)
@@ -319,7 +322,8 @@ def main():
                  else "synthetic (seeded phantom)"),
         "config": {"workload": w.name, "grid": [nx, ny, nz], "cells": cells, "materials": int(w.table.shape[0]),
                    "sources": int(w.sources.shape[0]), "probes": int(w.probes.shape[0]),
-                   "slice_every": w.slice_every, "launch": shape, "autotuned": tuned is not None,
+                   "slice_every": w.slice_every, "launch": shape, "xcd_remap": remap, "time_steps": a.time_steps,
+                   "autotuned": tuned is not None,
                    "timed_steps": [first, first + a.steps], "untimed_advance": a.advance,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

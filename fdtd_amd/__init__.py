@@ -275,14 +275,16 @@ def make_params(shape, dt, dx, dy=None, dz=None, mu0=None, eps0=None):
 class Context:
     """One grid's device state and the fused step (fdtd_amd.h)."""
 
-    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=0,
+    def __init__(self, params, debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, tile_y=0, kchunk=0, xcd_remap=None,
                  storage=STORE_F32, scales=(0.0, 0.0, 0.0), lane_cells=0, tile_x=0, load_zero_terms=0,
                  time_steps=0):
         L = lib()
         ext = FdtdExt()
         L.fdtd_ext_defaults(C.byref(ext))
         ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
-        ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
+        ext.tile_y, ext.kchunk = tile_y, kchunk
+        if xcd_remap is not None:      # else the library's default order
+            ext.xcd_remap = xcd_remap
         ext.storage = storage
         ext.scale_e, ext.scale_d, ext.scale_h = scales
         ext.lane_cells = lane_cells
@@ -471,11 +473,13 @@ class Context:
 
 
 def make_ext(debye=DEBYE_FIXED, jz_mode=JZ_STEP, jz_limit=0, storage=STORE_F32, scales=(0.0, 0.0, 0.0),
-             tile_y=0, kchunk=0, xcd_remap=0, load_zero_terms=0, time_steps=0):
+             tile_y=0, kchunk=0, xcd_remap=None, load_zero_terms=0, time_steps=0):
     ext = FdtdExt()
     lib().fdtd_ext_defaults(C.byref(ext))
     ext.debye_mode, ext.jz_index_mode, ext.jz_limit = debye, jz_mode, jz_limit
-    ext.tile_y, ext.kchunk, ext.xcd_remap = tile_y, kchunk, xcd_remap
+    ext.tile_y, ext.kchunk = tile_y, kchunk
+    if xcd_remap is not None:
+        ext.xcd_remap = xcd_remap
     ext.storage = storage
     ext.scale_e, ext.scale_d, ext.scale_h = scales
     ext.load_zero_terms = load_zero_terms

@@ -246,7 +246,7 @@ int fdtd_check_ext(const FdtdExt &e)
         return fdtd_fail(FDTD_EINVAL, "jz_index_mode %d", e.jz_index_mode);
     if (e.tile_y != 0 && e.tile_y != 2 && e.tile_y != 4 && e.tile_y != 8) return fdtd_fail(FDTD_EINVAL, "tile_y %d", e.tile_y);
     if (e.kchunk < 0 || e.jz_limit < 0) return fdtd_fail(FDTD_EINVAL, "negative kchunk/jz_limit");
-    if (e.xcd_remap < 0 || e.xcd_remap > 2) return fdtd_fail(FDTD_EINVAL, "xcd_remap %d", e.xcd_remap);
+    if (e.xcd_remap < 0 || e.xcd_remap > 4) return fdtd_fail(FDTD_EINVAL, "xcd_remap %d", e.xcd_remap);
     if (e.storage != FDTD_STORE_F32 && e.storage != FDTD_STORE_F16 && e.storage != FDTD_STORE_BF16 &&
         e.storage != FDTD_STORE_F32_FAST)
         return fdtd_fail(FDTD_EINVAL, "storage %d", e.storage);

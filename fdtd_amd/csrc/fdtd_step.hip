@@ -41,11 +41,16 @@ __device__ inline int xcd_logical(int b, int nblk, int mode, int unit)
         const int x = b % 8, s = b / 8;
         return x * q + min(x, rem) + s;
     }
-    if (mode == 2) {
-        // Groups of 8 units (a unit = one z-chunk layer of tiles): XCD x
-        // takes unit x of each group, so the 8 XCDs work on 8 adjacent
-        // layers while the tiles of a layer (which share halo rows and
-        // columns) share one L2.  A partial last group keeps the identity.
+    if (mode >= 2) {
+        // Groups of 8 units: XCD x takes unit x of each group.  Mode 2: a
+        // unit is one z-chunk layer of tiles, so the 8 XCDs work on 8
+        // adjacent layers while the tiles of a layer (which share halo rows
+        // and columns) share one L2.  Mode 3: a unit is one row of tiles
+        // along x, so the tiles that share x-halo columns share an L2 and
+        // the 8 XCDs work on adjacent rows of one layer.  Mode 4: a unit is
+        // an eighth of a layer's rows (a strip), so the XCDs split each
+        // layer and x and y neighbours share an L2 inside a strip.  A
+        // partial last group keeps the identity.
         const int G = 8 * unit;
         const int full = (nblk / G) * G;
         if (b >= full) return b;
@@ -53,6 +58,12 @@ __device__ inline int xcd_logical(int b, int nblk, int mode, int unit)
         return g * G + (r % 8) * unit + r / 8;
     }
     return b;
+}
+
+// Tiles per unit of xcd_logical's modes 2-4.
+__device__ inline int xcd_unit(int mode, int tiles_x, int tiles_y)
+{
+    return mode == 3 ? tiles_x : mode == 4 ? (tiles_y + 7) / 8 * tiles_x : tiles_x * tiles_y;
 }
 
 __device__ inline float e_update(float c, float d, float a, double g1, double g2, double g3, float e0, float e1,
@@ -911,7 +922,7 @@ __global__ __launch_bounds__(TXW *TY) __attribute__((amdgpu_waves_per_eu(5))) vo
 {
     const int tid = threadIdx.y * TXW + threadIdx.x;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
-    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, xcd_unit(A.xcd_remap, A.tiles_x, A.tiles_y));
     const int bx = b % A.tiles_x;
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);
@@ -1319,7 +1330,7 @@ __global__ __launch_bounds__(64 * R) __attribute__((amdgpu_waves_per_eu(4))) voi
     const StepArgs &A = S.A;
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
-    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, xcd_unit(A.xcd_remap, A.tiles_x, A.tiles_y));
     const int bx = b % A.tiles_x;
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);
@@ -1339,7 +1350,7 @@ __global__ __launch_bounds__(TX *TY) void fused_step_pair(const StepArgs A)
 {
     const int tid = threadIdx.y * TX + threadIdx.x;
     const int nblk = A.tiles_x * A.tiles_y * A.tiles_z;
-    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, A.tiles_x * A.tiles_y);
+    const int b = xcd_logical((int)blockIdx.x, nblk, A.xcd_remap, xcd_unit(A.xcd_remap, A.tiles_x, A.tiles_y));
     const int bx = b % A.tiles_x;
     const int by = (b / A.tiles_x) % A.tiles_y;
     const int bz = b / (A.tiles_x * A.tiles_y);

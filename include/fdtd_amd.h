@@ -58,8 +58,9 @@ typedef struct
     int jz_limit;       /* jz entries used; 0 = the whole table, 512 = the reference's device table */
     int tile_y;         /* rows per fused tile: 0 = auto (4), else 2, 4 or 8 */
     int kchunk;         /* z planes per fused block: 0 = auto (4) */
-    int xcd_remap;      /* block -> tile order: 0 identity (default), 1 one contiguous run
-                           per XCD, 2 adjacent z-chunk layers on the 8 XCDs */
+    int xcd_remap;      /* block -> tile order: 0 identity, 1 one contiguous run per XCD, 2 adjacent
+                           z-chunk layers on the 8 XCDs, 3 (default) adjacent rows of tiles along x on
+                           the 8 XCDs, 4 each layer in 8 strips of rows, one per XCD */
     int storage;        /* FDTD_STORE_F32 (default), _F16, _BF16 or _F32_FAST */
     float scale_e;      /* 16-bit storage: E stored as E / scale_e (0 = auto) */
     float scale_d;      /* 16-bit storage: D stored as D / (eps0 scale_d) (0 = auto) */

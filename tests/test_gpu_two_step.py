@@ -38,7 +38,7 @@ def _probes(shape):
                        [1, nx - 1, ny - 1, nz - 1], [0, nx // 2, 1, 1]], np.int32)
 
 
-def run(shape, time_steps, steps, seed, kchunk=0, energy=0, slices=0, xcd_remap=0, probes=None):
+def run(shape, time_steps, steps, seed, kchunk=0, energy=0, slices=0, xcd_remap=None, probes=None):
     rng = np.random.default_rng(seed)
     ids, table = random_materials(rng, shape, 6)
     ctx = F.Context(F.make_params(shape, DT, DX, mu0=MU0, eps0=EPS0), kchunk=kchunk, time_steps=time_steps,
@@ -112,12 +112,16 @@ def test_two_steps_h_probes_fall_back():
     _same(a, b)
 
 
-@pytest.mark.parametrize("xcd_remap", [1, 2])
-def test_two_steps_block_orders(xcd_remap):
-    shape = (130, 40, 33)
-    a = run(shape, 2, 10, 4, kchunk=4, xcd_remap=xcd_remap)
-    b = run(shape, 1, 10, 4, kchunk=4)
-    _same(a, b)
+@pytest.mark.parametrize("xcd_remap", [0, 1, 2, 3, 4])
+def test_block_orders(xcd_remap):
+    """Every block -> tile order, with one and two steps per launch, against
+    the identity order (a permutation of tiles changes no result).  3 x 9
+    tiles per layer in 10 layers: every order has full groups of 8 units
+    and a partial last group."""
+    shape = (130, 33, 40)
+    ref = run(shape, 1, 10, 4, kchunk=4, xcd_remap=0)
+    for ts in (1, 2):
+        _same(run(shape, ts, 10, 4, kchunk=4, xcd_remap=xcd_remap), ref)
 
 
 def test_two_steps_against_the_transcription():

@@ -137,21 +137,27 @@ PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phanto
                                                           "fused_step_plain256_large_phantom.md")]
 
 
-def profiled_traffic(workload, storage, launch, loading=False):
+def profiled_traffic(workload, storage, launch, loading=False, order=0):
     """(HBM bytes per fused launch, profile path) measured for this workload
-    at this launch shape (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes,
-    scaled by a 1 GiB copy) in a committed profile, or (None, None).  The
-    kernel that loads E^{n-1} and D^{n-1} on every row (loading=True) has
-    its own sections ("... at <shape>, every row loading ...")."""
+    at this launch shape and block order (rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE passes, scaled by a 1 GiB copy) in a committed profile, or
+    (None, None).  Sections are headed "## <workload> at <shape>, <order>
+    order[, every row loading]..."; the kernel that loads E^{n-1} and
+    D^{n-1} on every row (loading=True) has its own sections."""
     import re
     if storage != "f32":
         return None, None
-    head = f"## {workload} at {launch}, every row loading" if loading else f"## {workload} at {launch}:"
+    names = {0: "identity order", 3: "row order"}
+    if order not in names:
+        return None, None
     for path in PROFILES:
         if not os.path.exists(path):
             continue
-        for sec in open(path).read().split(head)[1:]:
-            sec = sec.split("\n## ", 1)[0]
+        for sec in open(path).read().split("\n## ")[1:]:
+            head = sec.split("\n", 1)[0]
+            m = re.match(re.escape(f"{workload} at {launch}") + r"(?![\dx])(.*)", head)
+            if not m or names[order] not in m.group(1) or ("every row loading" in m.group(1)) != loading:
+                continue
             rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
             wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)
             if rd and wr:
@@ -300,11 +306,13 @@ def main():
     traffic, traffic_src = None, None
     if a.traffic_kib:
         traffic, traffic_src = int((a.traffic_kib[0] + a.traffic_kib[1]) * 1024), "--traffic-kib"
+    elif per_launch != 1:
+        traffic, traffic_src = None, "no counter profile of the two-step kernel under profiles/"
     else:
-        traffic, prof = profiled_traffic(w.name, a.storage, launch, bool(a.load_zero_terms))
+        traffic, prof = profiled_traffic(w.name, a.storage, launch, bool(a.load_zero_terms), remap)
         traffic_src = (os.path.relpath(prof, ROOT) + f" (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, 1 GiB copy "
-                       f"scale; {launch} launch)") if traffic is not None else \
-            f"no counter profile of {w.name} at launch {launch}" + \
+                       f"scale; {launch} launch, block order {remap})") if traffic is not None else \
+            f"no counter profile of {w.name} at launch {launch}, block order {remap}" + \
             (" (every row loading)" if a.load_zero_terms else "") + " under profiles/"
     out = {
         "metric": "Gcell-updates/s (fused FDTD step, Debye media, Mur ABC)",

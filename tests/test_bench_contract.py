@@ -21,26 +21,34 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_profiled_traffic_matches_the_launch_shape():
-    """Each figure comes from the section of its own workload, launch shape and
-    kernel form (zero-term rows, or every row loading), wherever it sits."""
+    """Each figure comes from the section of its own workload, launch shape,
+    block order and kernel form (zero-term rows, or every row loading),
+    wherever it sits."""
     alg = 97 * 320 * 320 * 880
-    # the headline workload at the library's default launch shape has its own counter passes
-    tl, pl = bench.profiled_traffic("large_phantom", "f32", "64x4x16")
-    assert tl and 1.0 < tl / alg < 1.8 and pl.endswith("fused_step_large_phantom.md")
+    # the headline workload at the library's default launch shape and block order has its own counter passes
+    tl, pl = bench.profiled_traffic("large_phantom", "f32", "64x4x16", order=3)
+    assert tl and 1.0 < tl / alg < 1.1 and pl.endswith("fused_step_large_phantom.md")
+    # the identity order re-reads halo columns on other XCDs
+    ti, pi = bench.profiled_traffic("large_phantom", "f32", "64x4x16", order=0)
+    assert ti and 1.2 < ti / alg < 1.4 and pi == pl
     # 4-plane chunks read more (a z-chunk prologue per 4 planes), the loading kernel more still
     t4, _ = bench.profiled_traffic("large_phantom", "f32", "64x4x4")
     t4l, _ = bench.profiled_traffic("large_phantom", "f32", "64x4x4", loading=True)
-    assert t4 and t4l and tl < t4 < t4l < 1.8 * alg
-    # the 256^3 counters were taken with the loading kernel only
+    assert t4 and t4l and ti < t4 < t4l < 1.8 * alg
+    assert bench.profiled_traffic("large_phantom", "f32", "64x4x4", order=3) == (None, None)
+    # the 256^3 counters of the autotuned shape were taken with the loading kernel only
     t, p = bench.profiled_traffic("plain_256", "f32", "128x2x6", loading=True)
     t0, _ = bench.profiled_traffic("plain_256", "f32", "64x4x4", loading=True)
     assert t and t0 and 1.3 < t / (97 * 256 ** 3) < 1.7 and t0 > t and p.endswith(".md")
     assert bench.profiled_traffic("plain_256", "f32", "128x2x6") == (None, None)
-    # 256^3 at its default shape, with zero-term rows
-    t8, p8 = bench.profiled_traffic("plain_256", "f32", "64x4x8")
-    assert t8 and 1.2 < t8 / (97 * 256 ** 3) < 1.7 and p8.endswith("fused_step_plain_256.md")
-    # a shape without counters, or 16-bit storage, gives none
+    # 256^3 at its default shape, with zero-term rows, in both block orders
+    t8, p8 = bench.profiled_traffic("plain_256", "f32", "64x4x8", order=3)
+    t8i, _ = bench.profiled_traffic("plain_256", "f32", "64x4x8", order=0)
+    assert t8 and t8i and 1.0 < t8 / (97 * 256 ** 3) < t8i / (97 * 256 ** 3) < 1.7
+    assert p8.endswith("fused_step_plain_256.md")
+    # a shape or order without counters, or 16-bit storage, gives none
     assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
+    assert bench.profiled_traffic("plain_256", "f32", "64x4x8", order=2) == (None, None)
     assert bench.profiled_traffic("plain_256", "f16", "128x2x6", loading=True) == (None, None)
 
 

@@ -205,6 +205,7 @@ BatchLaunch launch_of(fdtd_batch *b, int cls, int off)
     l.cls = &b->dev->cls[cls];
     l.sh = &b->dev->sh;
     l.off = off;
+    l.xcd_runs = kBatchXcdRun;
     return l;
 }
 

@@ -1410,7 +1410,16 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(PAIR ? 4
     const int total = C->total;
     const int step = as_const(L.sh)->step + L.off;
     int q = 0, staged = -1;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    // Tile `it` of the loop goes to block it % G, on XCD it % 8 (G a
+    // multiple of 8).  With runs of U tiles, XCD x takes run x of each group
+    // of 8 runs, so U consecutive tiles of the class's list share an L2; a
+    // partial last group keeps the identity.
+    const int U = L.xcd_runs;
+    const bool runs = U > 0 && G % 8 == 0;
+    const int full = runs ? total / (8 * U) * (8 * U) : 0;
+    for (int it = b; it < total; it += G) {
+        const int t = (runs && it < full) ? it / (8 * U) * (8 * U) + (it % 8) * U + (it % (8 * U)) / 8 : it;
         while (C->start[q + 1] <= t) ++q;
         const int id = C->idx[q];
         Const<BatchInst> &d = I[id];

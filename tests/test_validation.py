@@ -17,6 +17,9 @@ def _params(shape):
     (dict(lane_cells=3), "lane_cells"),
     (dict(storage=9), "storage"),
     (dict(xcd_remap=5), "xcd_remap"),
+    (dict(xcd_remap=-1), "xcd_remap"),
+    (dict(time_steps=3), "time_steps"),
+    (dict(time_steps=-1), "time_steps"),
     (dict(kchunk=-1), "kchunk"),
     (dict(jz_limit=-2), "jz_limit"),
     (dict(debye=7), "debye_mode"),
@@ -39,3 +42,11 @@ def test_grid_beyond_32bit_offsets_rejected():
     byte offsets; a grid of 2^30 elements or more is refused up front."""
     with pytest.raises(F.FdtdError, match=r"code -2: .*too large"):
         F.Context(_params((2048, 2048, 300)))
+
+
+def test_extension_defaults():
+    """The library's defaults as the Python layer sees them: the row block
+    order (xcd_remap 3) and one step per launch (time_steps 0)."""
+    e = F.make_ext()
+    assert e.xcd_remap == 3 and e.time_steps == 0 and e.load_zero_terms == 0 and e.storage == F.STORE_F32
+    assert F.make_ext(xcd_remap=0, time_steps=2).xcd_remap == 0

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import fdtd_amd as F
+from fdtd_amd import workloads as W
 from oracle import oracle as O
 from tests.helpers import EPS0, MU0, first_diff, random_materials, random_rp, random_sources
 
@@ -153,3 +154,32 @@ def test_two_steps_against_the_transcription():
         x, y = ctx.get(f), sim.get(f)
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{F.FIELDS[f]}: {first_diff(x, y)}"
     ctx.close()
+
+
+def test_two_steps_plain_256_full_horizon():
+    """The 256^3 Debye phantom over its full 2000 steps (3 Ez probes, a
+    z-slice every 100 steps, which puts one single step in every 100): two
+    steps per launch against one, every field, level, probe and slice."""
+    steps = 2000
+    w = W.plain_256(steps=steps)
+
+    def go(ts):
+        ctx = F.Context(w.params, time_steps=ts)
+        ctx.set_materials(w.ids, w.table)
+        ctx.set_sources(w.sources, w.jz)
+        ctx.set_probes(w.probes, steps)
+        ctx.set_slices(w.slice_k, w.slice_every, steps // w.slice_every)
+        ctx.set_timing(1)
+        ctx.run(steps)
+        out = dict(fields={(f, lv): ctx.get(f, lv) for f in range(9) for lv in ([0] if 3 <= f < 6 else [0, 1, 2])},
+                   probes=ctx.probes(), n=ctx.n, timed_steps=ctx.timed_steps(), slices=ctx.slices())
+        ctx.close()
+        return out
+
+    print("\nplain_256, 2000 steps, two-step against one-step", flush=True)
+    a, b = go(2), go(1)
+    assert a["timed_steps"] == 2
+    _same(a, b)
+    assert np.abs(a["probes"]).max() > 0 and np.isfinite(a["probes"]).all()
+    assert list(a["slices"][0]) == list(b["slices"][0]) == list(range(100, steps + 1, 100))
+    assert np.array_equal(a["slices"][1].view(np.uint32), b["slices"][1].view(np.uint32))

@@ -1,4 +1,4 @@
-"""The two large BASELINE grids at full size against the FP32 transcription
+"""The two large benchmark grids at full size against the FP32 transcription
 of the reference, bit for bit, in every field of the newest level and in
 every probe sample:
   * 320x320x880 (50 tissue ids, a 10-cell Jz dipole 4 cells outside the

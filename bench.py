@@ -134,7 +134,8 @@ def build_workload(name, steps, a=None):
 
 
 PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phantom.md", "fused_step_plain_256.md",
-                                                          "fused_step_plain256_large_phantom.md")]
+                                                          "fused_step_plain256_large_phantom.md",
+                                                          "fused_step_riken.md")]
 
 
 def profiled_traffic(workload, storage, launch, loading=False, order=0):

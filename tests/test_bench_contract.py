@@ -46,6 +46,9 @@ def test_profiled_traffic_matches_the_launch_shape():
     t8i, _ = bench.profiled_traffic("plain_256", "f32", "64x4x8", order=0)
     assert t8 and t8i and 1.0 < t8 / (97 * 256 ** 3) < t8i / (97 * 256 ** 3) < 1.7
     assert p8.endswith("fused_step_plain_256.md")
+    # the RIKEN phantom at its default shape
+    tr, pr = bench.profiled_traffic("riken", "f32", "64x4x16", order=3)
+    assert tr and 1.0 < tr / (97 * 297 * 522 * 1719) < 1.2 and pr.endswith("fused_step_riken.md")
     # a shape or order without counters, or 16-bit storage, gives none
     assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
     assert bench.profiled_traffic("plain_256", "f32", "64x4x8", order=2) == (None, None)

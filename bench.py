@@ -135,7 +135,7 @@ def build_workload(name, steps, a=None):
 
 PROFILES = [os.path.join(ROOT, "profiles", f) for f in ("fused_step_large_phantom.md", "fused_step_plain_256.md",
                                                           "fused_step_plain256_large_phantom.md",
-                                                          "fused_step_riken.md")]
+                                                          "fused_step_riken.md", "fused_step_large_phantom_f16.md")]
 
 
 def profiled_traffic(workload, storage, launch, loading=False, order=0):
@@ -143,11 +143,10 @@ def profiled_traffic(workload, storage, launch, loading=False, order=0):
     at this launch shape and block order (rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE passes, scaled by a 1 GiB copy) in a committed profile, or
     (None, None).  Sections are headed "## <workload> at <shape>, <order>
-    order[, every row loading]..."; the kernel that loads E^{n-1} and
-    D^{n-1} on every row (loading=True) has its own sections."""
+    order[, <storage> storage][, every row loading]..."; the kernel that
+    loads E^{n-1} and D^{n-1} on every row (loading=True) has its own
+    sections."""
     import re
-    if storage != "f32":
-        return None, None
     names = {0: "identity order", 3: "row order"}
     if order not in names:
         return None, None
@@ -158,6 +157,9 @@ def profiled_traffic(workload, storage, launch, loading=False, order=0):
             head = sec.split("\n", 1)[0]
             m = re.match(re.escape(f"{workload} at {launch}") + r"(?![\dx])(.*)", head)
             if not m or names[order] not in m.group(1) or ("every row loading" in m.group(1)) != loading:
+                continue
+            # fp32 sections name no storage; 16-bit ones say "<storage> storage"
+            if (storage != "f32") != ("storage" in m.group(1)) or (storage != "f32" and f"{storage} storage" not in m.group(1)):
                 continue
             rd = re.search(r"measured HBM read \(scaled FETCH_SIZE\) \| [\d.]+ GB \((\d+) KiB x (\d+)\)", sec)
             wr = re.search(r"measured HBM write \(scaled WRITE_SIZE\) \| [\d.]+ GB \((\d+) KiB\)", sec)

@@ -53,6 +53,11 @@ def test_profiled_traffic_matches_the_launch_shape():
     assert bench.profiled_traffic("plain_256", "f32", "64x2x3") == (None, None)
     assert bench.profiled_traffic("plain_256", "f32", "64x4x8", order=2) == (None, None)
     assert bench.profiled_traffic("plain_256", "f16", "128x2x6", loading=True) == (None, None)
+    # fp16 storage at 320x320x880 has its own passes (49 B/cell model); bf16 has none
+    t16, p16 = bench.profiled_traffic("large_phantom", "f16", "64x4x16", order=3)
+    assert t16 and 1.1 < t16 / (49 * 320 * 320 * 880) < 1.4 and p16.endswith("fused_step_large_phantom_f16.md")
+    assert t16 != tl
+    assert bench.profiled_traffic("large_phantom", "bf16", "64x4x16", order=3) == (None, None)
 
 
 def test_headline_defaults():

@@ -8,6 +8,8 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
     python -m tests.long_runs bounds   # peak fields of soft Jz sources against the accumulated
                                        # and the largest kick (the basis of the 16-bit scales;
                                        # oracle only, no GPU)
+    python -m tests.long_runs two_step # 320x320x880, 5000 steps: two steps per launch against
+                                       # one, every field, level, probe and slice (GPU only)
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -141,6 +143,34 @@ def large(steps=5000):
         print(f"{name} vs FP64: probes {probes}; plane Dz Hy Ez {pl}", flush=True)
 
 
+def two_step(steps=5000):
+    """large_phantom over `steps` with two steps per launch (time_steps = 2)
+    and with one: every field of every ring level, the probe series and the
+    slices must agree bit for bit."""
+    w = W.large_phantom(steps=steps)
+    out = []
+    for ts in (2, 1):
+        t0 = time.perf_counter()
+        ctx = F.Context(w.params, time_steps=ts)
+        ctx.set_materials(w.ids, w.table)
+        ctx.set_sources(w.sources, w.jz)
+        ctx.set_probes(w.probes, steps)
+        ctx.set_slices(w.slice_k, w.slice_every, steps // w.slice_every)
+        ctx.set_timing(1)
+        ctx.run(steps)
+        fields = {(f, lv): ctx.get(f, lv) for f in range(9) for lv in ([0] if 3 <= f < 6 else [0, 1, 2])}
+        out.append((fields, ctx.probes(), ctx.slices(), ctx.timed_steps()))
+        ctx.close()
+        print(f"  time_steps {ts}: {steps} steps, {time.perf_counter() - t0:.1f} s", flush=True)
+    (fa, pa, sa, ka), (fb, pb, sb, kb) = out
+    same_fields = all(np.array_equal(fa[key].view(np.uint32), fb[key].view(np.uint32)) for key in fa)
+    same_probes = np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
+    same_slices = list(sa[0]) == list(sb[0]) and np.array_equal(sa[1].view(np.uint32), sb[1].view(np.uint32))
+    print(f"two-step launches ({ka} steps per timed launch) against one ({kb}): fields {same_fields}, "
+          f"probes {same_probes}, slices {same_slices} ({len(sa[0])} slices); probe peak "
+          f"{np.abs(pa).max():.3e}, finite {bool(np.isfinite(pa).all())}", flush=True)
+
+
 def bounds(n=40):
     """FP64 oracle in an n^3 box: the largest |E|, |D|/eps0 and eta0 |H| a
     soft Jz source drives, over K = max_t |sum jz| / eps0 (the accumulated
@@ -174,4 +204,5 @@ def bounds(n=40):
 
 if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
-    {"small": small, "served": served, "large": large, "bounds": bounds}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+    {"small": small, "served": served, "large": large, "bounds": bounds,
+     "two_step": two_step}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

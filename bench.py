@@ -169,18 +169,19 @@ def profiled_traffic(workload, storage, launch, loading=False, order=0):
 
 
 def zero_term_fraction(w, shape):
-    """Fraction of the fused kernel's (64-cell row segment, plane) units that
-    skip loading E^{n-1} and D^{n-1}: every cell of the segment has tau = 0
-    (exactly +0), and the kernel has decided for the plane (a z-chunk's first
-    plane always loads, and the bottom chunk decides from its third plane)."""
+    """Fraction of the fused kernel's (32-cell half-row segment, plane) units
+    that skip loading E^{n-1} and D^{n-1}: every cell of the segment (half a
+    wave, one 128-byte line of each array) has tau = 0 (exactly +0), and the
+    kernel has decided for the plane (a z-chunk's first plane always loads,
+    and the bottom chunk decides from its third plane)."""
     nx, ny, nz = w.shape
     kc = shape["kchunk"]
     zero_mat = np.asarray([np.float32(t[3]).view(np.uint32) == 0 for t in w.table])
     zc = zero_mat[w.ids]                                   # (nz, ny, nx)
-    segs = (nx + 63) // 64
-    pad = np.ones((nz, ny, segs * 64), bool)
+    segs = (nx + 31) // 32
+    pad = np.ones((nz, ny, segs * 32), bool)
     pad[:, :, :nx] = zc
-    seg_ok = pad.reshape(nz, ny, segs, 64).all(axis=3)     # (nz, ny, segs)
+    seg_ok = pad.reshape(nz, ny, segs, 32).all(axis=3)     # (nz, ny, segs)
     k = np.arange(nz)
     decided = ((k % kc) != 0) & ~((k < kc) & (k == 1))
     return float((seg_ok & decided[:, None, None]).sum() / seg_ok.size)

@@ -326,8 +326,9 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     };
     // Zero-term rows (ZS, exact form): a material with tau = +0 has a = +0 and
     // g3 = +0, so its Debye update multiplies E^{n-1} and D^{n-1} by +0.  When
-    // every own cell of the wave's row has such a material, the row's six
-    // E^{n-1}/D^{n-1} loads are skipped (wave-uniform).  The ids move two
+    // every own cell of a half wave (32 cells, one 128-byte line of each fp32
+    // array) has such a material, its six E^{n-1}/D^{n-1} loads are skipped
+    // (uniform per half: the other half's loads fetch only their own lines).  The ids move two
     // planes ahead: the id load of plane k+2 is issued first in iteration k
     // and decided at its end, so plane k+2's loads go out at once in
     // iteration k+1.  The first z-chunk decides from its third plane on (no
@@ -337,7 +338,8 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     auto zero_row = [&](uint32_t id) -> bool {
         const int m = id & kMaterialMask;
         const bool z = __float_as_uint(lA[m]) == 0u && __double_as_longlong(lG3[m]) == 0ll;   // both +0
-        return __all(!own || z);
+        const uint64_t b = __ballot(!own || z);
+        return ((b >> (tx & 32)) & 0xffffffffull) == 0xffffffffull;
     };
 
     // Mur x faces: the tile's coefficients (planes k0..k1-1, rows j0..j0+TY-1,

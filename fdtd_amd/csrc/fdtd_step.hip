@@ -328,12 +328,12 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
     // g3 = +0, so its Debye update multiplies E^{n-1} and D^{n-1} by +0.  When
     // every own cell of a half wave (32 cells, one 128-byte line of each fp32
     // array) has such a material, its six E^{n-1}/D^{n-1} loads are skipped
-    // (uniform per half: the other half's loads fetch only their own lines).  The ids move two
-    // planes ahead: the id load of plane k+2 is issued first in iteration k
-    // and decided at its end, so plane k+2's loads go out at once in
-    // iteration k+1.  The first z-chunk decides from its third plane on (no
-    // barrier guards the coefficient table earlier), the others from their
-    // second.
+    // (uniform per half: the other half's loads fetch only their own lines).
+    // The ids move two planes ahead: the id load of plane k+2 is issued first
+    // in iteration k and decided at its end, so plane k+2's loads go out at
+    // once in iteration k+1.  The first z-chunk decides from its third plane
+    // on (no barrier guards the coefficient table earlier), the others from
+    // their second.
     const bool zskip = ZS && A.zskip;
     auto zero_row = [&](uint32_t id) -> bool {
         const int m = id & kMaterialMask;
@@ -490,7 +490,8 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
                 // result is the reference's unless d*E^n is -0 (a zero sum then takes the
                 // sign of the dropped term, which may be -0) or E^n or D^n is not finite
                 // (E^{n-1} or D^{n-1} can only be infinite or NaN where they are, and
-                // 0*inf is NaN).  A row with such a cell loads the real operands after all.
+                // 0*inf is NaN).  If any skipping lane of the wave holds such a cell, every
+                // skipping lane loads the real operands after all.
                 auto neg0 = [](float v) { return __float_as_uint(v) == 0x80000000u; };
                 auto fin = [](float v) { return __builtin_isfinite(v); };
                 const bool risky = own && (neg0(cd * exk) || neg0(cd * eyk) || neg0(cd * ezk) || !fin(exk) ||

@@ -31,17 +31,20 @@
  *                         r^2 - c d r + c a = 0 (numpy.roots), FIXED steady
  *                         state recovers eps(omega) of a Debye medium.
  *   or_update_mur_*       reduces in vacuum to the textbook first-order Mur
- *                         form E0' = E1 + (S-1)/(S+1)(E1' - E0); absorbed
- *                         energy vs a closed box.
+ *                         form E0' = E1 + (S-1)/(S+1)(E1' - E0); with rp
+ *                         varying cell by cell, the residual of the one-way
+ *                         wave equation with each node at its own speed
+ *                         (fixes which rp plane is the face's, p0, and which
+ *                         the neighbour's, p1); absorbed energy vs a closed
+ *                         box.
  *   or_update_sources_*   linearity in jz and the 3-step sample hold.
  *   or_load_specs/pgm     the tissue table fixture (53 rows) and a PGM file in
  *                         the reference's layout.
  * Parity unpinned: nvcc's exact FMA contraction on the reference device code
- * (no NVIDIA GPU or nvcc exists here); which rp plane is the face (p0) and
- * which the neighbour (p1) in or_update_mur_*, and where the FP64
- * promotions sit in or_update_e_f32 (both as read in the reference's text:
- * with rp uniform a swap is invisible, and a moved promotion changes results
- * by ulps, inside every FP64 bound); the reference's undefined behaviour
+ * (no NVIDIA GPU or nvcc exists here); where the FP64 promotions sit in
+ * or_update_e_f32 (as read in the reference's text: a moved promotion
+ * changes results by ulps, inside every FP64 bound); the reference's
+ * undefined behaviour
  * (out-of-range coefficient loads, specs[94], dJz past 512) is replaced by
  * defined behaviour: no out-of-range loads, code 94 stored as parsed, and
  * jz = 0 past the end of the table.

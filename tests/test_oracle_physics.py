@@ -179,6 +179,63 @@ def test_mur_is_textbook_first_order_in_vacuum():
         assert not changed[1:-1, 1:-1, 1:-1].any()  # interior untouched
 
 
+def test_mur_roles_of_the_rp_planes():
+    """With rp varying cell by cell, every face value satisfies Mur's
+    first-order one-way wave equation centred between the face cell (0) and
+    its neighbour (1) at t^{n+1/2}, each node's time difference taken at its
+    own speed v_i = c / sqrt(rp_i):
+        dt (E1' + E1 - E0' - E0) = s0 (E0' - E0) + s1 (E1' - E1),
+        s_i = h sqrt(mu0 eps0 rp_i).
+    That fixes which rp plane is the face's and which the neighbour's
+    (fdtd_calculations.cu:237-427); the same values with the roles swapped
+    leave residuals thousands of times larger.  The rp planes are filled
+    from a per-cell field the way the rp set-up fills them from materials
+    (index a is cell a on a low face and cell n-2+a on a high one)."""
+    rng = np.random.default_rng(11)
+    shape = (9, 8, 7)
+    nx, ny, nz = shape
+    g = _grid(shape)
+    tgt = [rng.uniform(-1, 1, shape[::-1]).astype(np.float32) for _ in range(3)]
+    src = [rng.uniform(-1, 1, shape[::-1]).astype(np.float32) for _ in range(3)]
+    before = [t.copy() for t in tgt]
+    rpc = rng.uniform(1.0, 60.0, shape[::-1]).astype(np.float32)          # (k, j, i)
+    rpx0 = np.ascontiguousarray(rpc[:, :, 0:2]).ravel()                   # [k][j][a], cell i = a
+    rpxE = np.ascontiguousarray(rpc[:, :, nx - 2:]).ravel()               # cell i = nx - 2 + a
+    rpy0 = np.ascontiguousarray(rpc[:, 0:2, :]).ravel()                   # [k][a][i]
+    rpyE = np.ascontiguousarray(rpc[:, ny - 2:, :]).ravel()
+    rpz0 = np.ascontiguousarray(rpc[0:2]).ravel()                         # [a][j][i]
+    rpzE = np.ascontiguousarray(rpc[nz - 2:]).ravel()
+    O.lib().or_update_mur_f32(C.byref(g), *tgt, *src, rpx0, rpy0, rpz0, rpxE, rpyE, rpzE)
+    m = np.float64(np.float32(MU0)) * np.float64(np.float32(EPS0))
+    slow = np.float64(DX) * np.sqrt(m * rpc.astype(np.float64))          # h / v per cell
+    dt = np.float64(DT)
+    I, J, K = slice(0, nx - 1), slice(1, ny - 1), slice(1, nz - 1)
+    Jy, Iy, Kz = slice(0, ny - 1), slice(1, nx - 1), slice(0, nz - 1)
+    # (field, face index, neighbour index, E1' from level n+1 or n): every face the update writes
+    faces = [(0, (K, 0, I), (K, 1, I), 1), (0, (K, -1, I), (K, -2, I), 1),
+             (0, (0, J, I), (1, J, I), 1), (0, (-1, J, I), (-2, J, I), 1),
+             (1, (K, Jy, 0), (K, Jy, 1), 1), (1, (K, Jy, -1), (K, Jy, -2), 0),   # x-high Ey: E1 twice
+             (1, (0, Jy, Iy), (1, Jy, Iy), 1), (1, (-1, Jy, Iy), (-2, Jy, Iy), 1),
+             (2, (Kz, J, 0), (Kz, J, 1), 1), (2, (Kz, J, -1), (Kz, J, -2), 1),
+             (2, (Kz, 0, Iy), (Kz, 1, Iy), 1), (2, (Kz, -1, Iy), (Kz, -2, Iy), 1)]
+    worst, worst_swapped = 0.0, np.inf
+    for q, f0, f1, new in faces:
+        e0n, e1n = tgt[q][f0].astype(np.float64), (tgt if new else src)[q][f1].astype(np.float64)
+        e0, e1 = src[q][f0].astype(np.float64), src[q][f1].astype(np.float64)
+        s0, s1 = slow[f0], slow[f1]
+        scale = (dt + s0 + s1) * 2.0
+        res = dt * (e1n + e1 - e0n - e0) - s0 * (e0n - e0) - s1 * (e1n - e1)
+        worst = max(worst, float(np.max(np.abs(res) / scale)))
+        # the same face value with the planes' roles swapped, checked against the same equation
+        sw = ((dt - s0) * e1n + (dt + s0) * e1 - (dt - s1) * e0) / (dt + s1)
+        res_sw = dt * (e1n + e1 - sw - e0) - s0 * (sw - e0) - s1 * (e1n - e1)
+        worst_swapped = min(worst_swapped, float(np.max(np.abs(res_sw) / scale)))
+    assert worst < 2e-6, worst
+    assert worst_swapped > 1e3 * worst and worst_swapped > 1e-3, (worst, worst_swapped)
+    for q in range(3):
+        assert not (tgt[q] != before[q])[1:-1, 1:-1, 1:-1].any()  # interior untouched
+
+
 def _pulse_run(mur, steps, shape=(60, 60, 60)):
     nx, ny, nz = shape
     s = O.Sim(shape, DT, DX, DX, DX, MU0, EPS0, prec=64, mur=mur)

@@ -10,6 +10,8 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
                                        # oracle only, no GPU)
     python -m tests.long_runs two_step # 320x320x880, 5000 steps: two steps per launch against
                                        # one, every field, level, probe and slice (GPU only)
+    python -m tests.long_runs zero_terms  # 320x320x880, 5000 steps: zero-term half rows against
+                                          # every row loading, bit for bit (GPU only)
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -143,15 +145,14 @@ def large(steps=5000):
         print(f"{name} vs FP64: probes {probes}; plane Dz Hy Ez {pl}", flush=True)
 
 
-def two_step(steps=5000):
-    """large_phantom over `steps` with two steps per launch (time_steps = 2)
-    and with one: every field of every ring level, the probe series and the
-    slices must agree bit for bit."""
-    w = W.large_phantom(steps=steps)
+def _same_runs(w, steps, variants):
+    """Runs w over `steps` in each of two context variants (keyword
+    arguments of F.Context) and reports whether every field of every ring
+    level, the probe series and the slices agree bit for bit."""
     out = []
-    for ts in (2, 1):
+    for kw in variants:
         t0 = time.perf_counter()
-        ctx = F.Context(w.params, time_steps=ts)
+        ctx = F.Context(w.params, **kw)
         ctx.set_materials(w.ids, w.table)
         ctx.set_sources(w.sources, w.jz)
         ctx.set_probes(w.probes, steps)
@@ -161,14 +162,35 @@ def two_step(steps=5000):
         fields = {(f, lv): ctx.get(f, lv) for f in range(9) for lv in ([0] if 3 <= f < 6 else [0, 1, 2])}
         out.append((fields, ctx.probes(), ctx.slices(), ctx.timed_steps()))
         ctx.close()
-        print(f"  time_steps {ts}: {steps} steps, {time.perf_counter() - t0:.1f} s", flush=True)
+        print(f"  {kw}: {steps} steps, {time.perf_counter() - t0:.1f} s", flush=True)
     (fa, pa, sa, ka), (fb, pb, sb, kb) = out
     same_fields = all(np.array_equal(fa[key].view(np.uint32), fb[key].view(np.uint32)) for key in fa)
     same_probes = np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
     same_slices = list(sa[0]) == list(sb[0]) and np.array_equal(sa[1].view(np.uint32), sb[1].view(np.uint32))
-    print(f"two-step launches ({ka} steps per timed launch) against one ({kb}): fields {same_fields}, "
-          f"probes {same_probes}, slices {same_slices} ({len(sa[0])} slices); probe peak "
-          f"{np.abs(pa).max():.3e}, finite {bool(np.isfinite(pa).all())}", flush=True)
+    return dict(fields=same_fields, probes=same_probes, slices=same_slices, nslices=len(sa[0]),
+                steps_per_launch=(ka, kb), peak=float(np.abs(pa).max()), finite=bool(np.isfinite(pa).all()))
+
+
+def two_step(steps=5000):
+    """large_phantom over `steps` with two steps per launch (time_steps = 2)
+    and with one: every field of every ring level, the probe series and the
+    slices must agree bit for bit."""
+    r = _same_runs(W.large_phantom(steps=steps), steps, [dict(time_steps=2), dict(time_steps=1)])
+    ka, kb = r["steps_per_launch"]
+    print(f"two-step launches ({ka} steps per timed launch) against one ({kb}): fields {r['fields']}, "
+          f"probes {r['probes']}, slices {r['slices']} ({r['nslices']} slices); probe peak "
+          f"{r['peak']:.3e}, finite {r['finite']}", flush=True)
+
+
+def zero_terms(steps=5000):
+    """large_phantom over `steps` with zero-term half rows (the default) and
+    with every row loading E^{n-1} and D^{n-1} (load_zero_terms = 1): every
+    field of every ring level, the probe series and the slices must agree
+    bit for bit, through the late field's small and -0 values."""
+    r = _same_runs(W.large_phantom(steps=steps), steps, [dict(load_zero_terms=0), dict(load_zero_terms=1)])
+    print(f"zero-term half rows against loading every row: fields {r['fields']}, probes {r['probes']}, "
+          f"slices {r['slices']} ({r['nslices']} slices); probe peak {r['peak']:.3e}, finite {r['finite']}",
+          flush=True)
 
 
 def bounds(n=40):
@@ -205,4 +227,4 @@ def bounds(n=40):
 if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
     {"small": small, "served": served, "large": large, "bounds": bounds,
-     "two_step": two_step}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+     "two_step": two_step, "zero_terms": zero_terms}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

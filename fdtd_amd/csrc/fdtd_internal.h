@@ -66,9 +66,10 @@ struct StepArgs {
     const float4 *murx0, *murx1;    // per (k, j) coefficients of the x-low / x-high faces (mur_x_coefs)
     int cpl;                        // cells per lane: 1, or 2 for 16-bit storage (128-cell tiles)
     int txw;                        // threads per tile row with one cell per lane: 64 or 128
-    // Exact fp32 form: a wave row whose cells all have materials whose Debye
-    // update multiplies E^{n-1} and D^{n-1} by +0 (a = +0 and g3 = +0, i.e.
-    // tau = +0) does not load those six operands (see fused_tile).
+    // Exact fp32 form: a half wave (32 cells of a tile row) whose cells all
+    // have materials whose Debye update multiplies E^{n-1} and D^{n-1} by +0
+    // (a = +0 and g3 = +0, i.e. tau = +0) does not load those six operands
+    // (see fused_tile).
     int zskip;
 };
 

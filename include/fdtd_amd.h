@@ -68,10 +68,11 @@ typedef struct
     int lane_cells;     /* 16-bit storage: cells per lane of the fused kernel (1 or 2; 0 = auto: 2 unless
                            128-cell tiles leave clearly more lanes idle than 64-cell ones) */
     int tile_x;         /* threads per tile row with one cell per lane: 0 = 64, or 64, 128 */
-    int load_zero_terms; /* fp32 storage: 0 (default) skips loading E^{n-1} and D^{n-1} on tile rows whose
-                            cells' Debye update multiplies them by exactly zero (tau = 0: vacuum and
-                            non-dispersive media), with results unchanged bit for bit; 1 always loads
-                            them.  Off after fdtd_ctx_set_field or fdtd_ctx_load. */
+    int load_zero_terms; /* fp32 storage: 0 (default) skips loading E^{n-1} and D^{n-1} on the half rows
+                            of a wave (32 cells) whose cells' Debye update multiplies them by exactly
+                            zero (tau = 0: vacuum and non-dispersive media), with results unchanged bit
+                            for bit; 1 always loads them.  Off after fdtd_ctx_set_field or
+                            fdtd_ctx_load. */
     int time_steps;     /* fp32 storage: time steps per fused launch, 0 = auto (1) or 1, 2.  With 2, fdtd_run
                            advances pairs of steps in one launch where it can (stepped from zero, no halo
                            exchange, no H probes, no energy sum or slice on the first step of a pair),

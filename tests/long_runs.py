@@ -12,6 +12,8 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
                                        # one, every field, level, probe and slice (GPU only)
     python -m tests.long_runs zero_terms  # 320x320x880, 5000 steps: zero-term half rows against
                                           # every row loading, bit for bit (GPU only)
+    python -m tests.long_runs batched  # the 64-grid batched set over its full 527-2978 steps in one
+                                       # batch against single contexts, bit for bit (GPU only)
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -193,6 +195,40 @@ def zero_terms(steps=5000):
           flush=True)
 
 
+def batched():
+    """All 64 grids of the batched set over their full step counts
+    (527-2978) in one batch (autotuned shape, graphs, runs of tiles per
+    XCD), against each grid stepped alone in a fused context: every probe
+    sample and every field after the last step, bit for bit."""
+    ws = W.batched_set()
+    insts = [F.Instance(w.params, F.make_ext(), w.ids, w.table, w.sources, w.jz, w.steps, w.probes) for w in ws]
+    arena = int(sum(x.nbytes() for x in insts) * 1.02) + (64 << 20)
+    t0 = time.perf_counter()
+    with F.Batch(arena) as b:
+        ids = [b.add(x) for x in insts]
+        ty, kc = b.autotune(trial_steps=2)
+        b.run(max(w.steps for w in ws))
+        assert b.info()["running"] == 0
+        got = [(b.probes(i), [b.get(i, f) for f in range(9)]) for i in ids]
+    print(f"  batch: 64 grids, {ty} tile rows, {kc} planes, {time.perf_counter() - t0:.1f} s", flush=True)
+    bad = []
+    for q, (w, (pr, fields)) in enumerate(zip(ws, got)):
+        ctx = F.Context(w.params)
+        ctx.set_materials(w.ids, w.table)
+        ctx.set_sources(w.sources, w.jz)
+        ctx.set_probes(w.probes, w.steps)
+        ctx.run(w.steps)
+        same = np.array_equal(pr.view(np.uint64), ctx.probes().view(np.uint64)) and pr.shape[0] == w.steps
+        same = same and all(np.array_equal(np.ascontiguousarray(fields[f], np.float32).view(np.uint32),
+                                           np.ascontiguousarray(ctx.get(f), np.float32).view(np.uint32))
+                            for f in range(9))
+        ctx.close()
+        if not same:
+            bad.append((q, w.shape))
+    print(f"batched set, full horizons, batch against single contexts: {64 - len(bad)} of 64 grids bit-equal "
+          f"(probes and all nine fields); differing: {bad}", flush=True)
+
+
 def bounds(n=40):
     """FP64 oracle in an n^3 box: the largest |E|, |D|/eps0 and eta0 |H| a
     soft Jz source drives, over K = max_t |sum jz| / eps0 (the accumulated
@@ -227,4 +263,4 @@ def bounds(n=40):
 if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
     {"small": small, "served": served, "large": large, "bounds": bounds,
-     "two_step": two_step, "zero_terms": zero_terms}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+     "two_step": two_step, "zero_terms": zero_terms, "batched": batched}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

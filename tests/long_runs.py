@@ -14,8 +14,9 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
                                           # every row loading, bit for bit (GPU only)
     python -m tests.long_runs batched  # the 64-grid batched set over its full 527-2978 steps in one
                                        # batch against single contexts, bit for bit (GPU only)
-    python -m tests.long_runs served_full  # the 120-request served stream, fp32, full lengths, at
-                                           # its arrival times, against single contexts (GPU only)
+    python -m tests.long_runs served_full  # the 120-request served stream at full lengths and its
+                                           # arrival times, all fp32 and then half fp16, against
+                                           # single contexts (GPU only)
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -23,6 +24,7 @@ thing measured.
 import os
 import sys
 import time
+import warnings
 
 import numpy as np
 
@@ -232,40 +234,46 @@ def batched():
 
 
 def served_full(count=120):
-    """The served stream, every request fp32 and at its full step count
-    (200-4000), submitted at its arrival time to one server; each request's
-    probe series against the same grid stepped alone in a fused context,
-    bit for bit."""
-    stream = W.served_stream(count, f16_share=0.0)
-    insts = [F.Instance(w.params, F.make_ext(), w.ids, w.table, w.sources, w.jz, w.steps, w.probes)
-             for _, w, _ in stream]
-    with F.Server(200 << 30, max_slots=64, segment_steps=8) as srv:
-        t0 = time.perf_counter()
-        rs = []
-        for (t, _, _), x in zip(stream, insts):
-            d = t - (time.perf_counter() - t0)
-            if d > 0:
-                time.sleep(d)
-            rs.append(srv.submit(x))
-        assert srv.drain(600.0)
-        st = srv.stats()
-        got = [srv.result(r) for r in rs]
-    print(f"  server: {st['completed']} solves, {st['solves_per_s']:.1f} solves/s, peak running "
-          f"{st['peak_running']}", flush=True)
-    bad = []
-    for q, ((_, w, _), (series, info)) in enumerate(zip(stream, got)):
-        ctx = F.Context(w.params)
-        ctx.set_materials(w.ids, w.table)
-        ctx.set_sources(w.sources, w.jz)
-        ctx.set_probes(w.probes, w.steps)
-        ctx.run(w.steps)
-        want = ctx.probes()
-        ctx.close()
-        if info["state"] != F.REQ_DONE or series.shape != want.shape or \
-                not np.array_equal(series.view(np.uint64), want.view(np.uint64)):
-            bad.append((q, w.shape))
-    print(f"served stream, {count} fp32 requests at full length, server against single contexts: "
-          f"{count - len(bad)} of {count} probe series bit-equal; differing: {bad}", flush=True)
+    """The served stream at every request's full step count (200-4000),
+    submitted at its arrival time to one server: all fp32, then with about
+    half the requests in fp16 storage (the default stream).  Each request's
+    probe series against the same grid stepped alone in a fused context of
+    its storage, bit for bit."""
+    for share in (0.0, 0.5):
+        stream = W.served_stream(count, f16_share=share)
+        store = [F.STORE_F16 if st == "f16" else F.STORE_F32 for _, _, st in stream]
+        insts = [F.Instance(w.params, F.make_ext(storage=sto), w.ids, w.table, w.sources, w.jz, w.steps, w.probes)
+                 for (_, w, _), sto in zip(stream, store)]
+        with F.Server(200 << 30, max_slots=64, segment_steps=8) as srv:
+            t0 = time.perf_counter()
+            rs = []
+            for (t, _, _), x in zip(stream, insts):
+                d = t - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                rs.append(srv.submit(x))
+            assert srv.drain(600.0)
+            st = srv.stats()
+            got = [srv.result(r) for r in rs]
+        print(f"  server: {st['completed']} solves, {st['solves_per_s']:.1f} solves/s, peak running "
+              f"{st['peak_running']}", flush=True)
+        bad = []
+        for q, ((_, w, _), sto, (series, info)) in enumerate(zip(stream, store, got)):
+            ctx = F.Context(w.params, storage=sto)
+            ctx.set_materials(w.ids, w.table)
+            ctx.set_sources(w.sources, w.jz)
+            ctx.set_probes(w.probes, w.steps)
+            ctx.run(w.steps)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", F.PrecisionWarning)   # compared, not judged, here
+                want = ctx.probes()
+            ctx.close()
+            if info["state"] != F.REQ_DONE or series.shape != want.shape or \
+                    not np.array_equal(series.view(np.uint64), want.view(np.uint64)):
+                bad.append((q, w.shape))
+        nf16 = sum(sto == F.STORE_F16 for sto in store)
+        print(f"served stream, {count} requests ({nf16} fp16) at full length, server against single contexts: "
+              f"{count - len(bad)} of {count} probe series bit-equal; differing: {bad}", flush=True)
 
 
 def bounds(n=40):

@@ -17,6 +17,8 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
     python -m tests.long_runs served_full  # the 120-request served stream at full lengths and its
                                            # arrival times, all fp32 and then half fp16, against
                                            # single contexts (GPU only)
+    python -m tests.long_runs riken_body   # the RIKEN phantom at body scale, 200 steps, against the
+                                           # FP32 transcription, every field and probe
 
 Each prints one line per case.  The oracle is the checker here, never the
 thing measured.
@@ -276,6 +278,32 @@ def served_full(count=120):
               f"{count - len(bad)} of {count} probe series bit-equal; differing: {bad}", flush=True)
 
 
+def riken_body(steps=200):
+    """The RIKEN phantom at body scale (297x522x1719 with its vacuum padding,
+    266.5 M cells) over `steps`: the fused context at its defaults against
+    the oracle's FP32 transcription, every probe sample and every field, bit
+    for bit."""
+    from fdtd_amd import phantom as P
+    table, _ = F.load_tissue_table(os.path.join(ROOT, "tests", "golden", "mat_specs_riken"))
+    w = W.riken(P.load(os.path.join(ROOT, "data", "riken_ids.npz"), P.RIKEN_SHAPE), table, steps=steps)
+    ctx = gpu_ctx(w, F.STORE_F32, steps)
+    ctx.run(steps)
+    s = oracle_sim(w, 32)
+    series, done, t0 = [], 0, time.perf_counter()
+    while done < steps:
+        n = min(20, steps - done)
+        series.append(s.run(n, w.probes))
+        done += n
+        print(f"  oracle: step {done}, {time.perf_counter() - t0:.0f} s", flush=True)
+    ref = np.concatenate(series)
+    same_probes = np.array_equal(ctx.probes().view(np.uint64), ref.view(np.uint64))
+    differ = [F.FIELDS[f] for f in range(9) if not np.array_equal(ctx.get(f).view(np.uint32), s.get(f).view(np.uint32))]
+    peak = float(np.abs(ref).max())
+    ctx.close()
+    print(f"riken {w.shape}, {steps} steps, GPU against the FP32 transcription: probes {same_probes} (peak {peak:.3e}), "
+          f"fields differing: {differ}", flush=True)
+
+
 def bounds(n=40):
     """FP64 oracle in an n^3 box: the largest |E|, |D|/eps0 and eta0 |H| a
     soft Jz source drives, over K = max_t |sum jz| / eps0 (the accumulated
@@ -311,4 +339,4 @@ if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
     {"small": small, "served": served, "large": large, "bounds": bounds,
      "two_step": two_step, "zero_terms": zero_terms, "batched": batched,
-     "served_full": served_full}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+     "served_full": served_full, "riken_body": riken_body}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

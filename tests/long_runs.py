@@ -5,6 +5,8 @@ hand on a GPU host (the oracle runs on the host's cores, up to 16 threads).
     python -m tests.long_runs served   # fp16 requests of the served stream vs FP64
     python -m tests.long_runs large    # 320x320x880, 5000 steps, fp32/fp16/bf16 vs FP64
                                        # and the FP32 transcription (about 25 min of host time)
+    python -m tests.long_runs large32  # the same, fp32 storage against the transcription only
+                                       # (about 11 min of host time)
     python -m tests.long_runs bounds   # peak fields of soft Jz sources against the accumulated
                                        # and the largest kick (the basis of the 16-bit scales;
                                        # oracle only, no GPU)
@@ -109,10 +111,11 @@ def served(count=10):
               f"cells away; centre/near rel-L2: " + " | ".join(errs), flush=True)
 
 
-def large(steps=5000):
+def large(steps=5000, fp64=True):
     """large_phantom over `steps`: probe series and the z = nz/2 planes of Dz,
     Hy, Ez at 1000, 2500 and `steps` against FP64 and the FP32
-    transcription."""
+    transcription (fp64=False: fp32 storage against the transcription
+    only)."""
     checks = tuple(c for c in (1000, 2500) if c < steps) + (steps,)
     w = W.large_phantom(steps=steps)
     k = w.shape[2] // 2
@@ -131,7 +134,7 @@ def large(steps=5000):
         return np.concatenate(series), planes
 
     gpu = {}
-    for name, st in STORAGES:
+    for name, st in (STORAGES if fp64 else STORAGES[:1]):
         ctx = gpu_ctx(w, st, steps)
         planes, done = {}, 0
         for c in checks:
@@ -145,6 +148,8 @@ def large(steps=5000):
     same = np.array_equal(gpu["fp32"][0], p32) and all(
         np.array_equal(gpu["fp32"][1][key].view(np.uint32), pl32[key].view(np.uint32)) for key in pl32)
     print(f"fp32 bit-equal to the FP32 transcription (probes and planes): {same}", flush=True)
+    if not fp64:
+        return
     p64, pl64 = run_oracle(64)
     for name, (p, planes) in gpu.items():
         probes = " ".join(f"{rel_l2(p[:, q], p64[:, q]):.1e}" for q in range(p.shape[1]))
@@ -339,4 +344,4 @@ if __name__ == "__main__":
     O.set_threads(min(os.cpu_count() or 1, 16))
     {"small": small, "served": served, "large": large, "bounds": bounds,
      "two_step": two_step, "zero_terms": zero_terms, "batched": batched,
-     "served_full": served_full, "riken_body": riken_body}[sys.argv[1] if len(sys.argv) > 1 else "small"]()
+     "served_full": served_full, "riken_body": riken_body, "large32": lambda: large(fp64=False)}[sys.argv[1] if len(sys.argv) > 1 else "small"]()

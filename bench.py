@@ -22,7 +22,8 @@ Exits with status 3 when the probes saw no finite non-zero signal.
 Two more workloads measure the systems patterns:
   --workload batched  64 uneven grids (batched_set) stepped together, one
                       persistent fused launch per step from captured graphs,
-                      shape picked by the autotuner; whole-job Gcell-updates/s
+                      shape picked by the autotuner (or fixed by --shape
+                      64xTYxKC); whole-job Gcell-updates/s
                       beside running the same grids one after another.
   --workload riken    the RIKEN voxel phantom at body scale (297x522x1719 with
                       its vacuum pad), from `--phantom` (a PGM directory or
@@ -111,7 +112,7 @@ def parse():
                     help="fp32: time steps per fused launch (2: the two-step kernel where it applies)")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape TXxTYxKC (threads per tile row, tile rows, z planes per block), "
-                         "e.g. 128x2x6; no autotuning")
+                         "e.g. 128x2x6; no autotuning (batched: 64xTYxKC)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
@@ -633,7 +634,14 @@ def bench_batched(a):
     t_upload = time.perf_counter() - t0
     stream = hip.Stream()
     t0 = time.perf_counter()
-    ty, kc = b.autotune(trial_steps=4, stream=stream.handle)
+    if a.shape:
+        # a fixed shape (counter passes need the same kernel in every pass): 64 x tile rows x planes
+        tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
+        if tx != 64:
+            raise SystemExit("batched: --shape takes 64xTYxKC (64-wide tiles)")
+        b.set_shape(tile_y=ty, kchunk=kc)
+    else:
+        ty, kc = b.autotune(trial_steps=4, stream=stream.handle)
     t_tune = time.perf_counter() - t0
     info = b.info()
     steps = max(w.steps for w in ws)

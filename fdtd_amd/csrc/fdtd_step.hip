@@ -80,7 +80,7 @@ __device__ inline float e_update(float c, float d, float a, double g1, double g2
 // storage types.  fp32 values pass through; fp16/bf16 convert to and from
 // fp32 with round-to-nearest-even.  The _nt forms are the streaming
 // (nontemporal) variants for data a launch touches once: the ring levels n
-// and n-1 of D, level n-1 of E, the ids and every store.
+// and n-1 of D, level n-1 of E and every store.
 template <typename T> struct Elem;
 
 template <> struct Elem<float> {
@@ -300,7 +300,9 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         uint32_t id;
         bool zs;
     };
-    auto fetch_id = [&](int k) -> uint32_t { return __builtin_nontemporal_load(ids + ((uint32_t)k * szb + c_own) / ES); };
+    // ids load cached: a wave's 64 ids are half of a 128-byte line whose other half the x-neighbour
+    // tile (on the same XCD in the row block order) reads
+    auto fetch_id = [&](int k) -> uint32_t { return ids[((uint32_t)k * szb + c_own) / ES]; };
     // load_id: the plane's ids come with its operands (else the caller loaded them ahead)
     auto fetch_own = [&](int k, Own &o, bool zs, bool load_id) {
         const uint32_t ob = (uint32_t)k * szb + c_own;
@@ -317,7 +319,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &A, int bx, int by, in
         }
         o.dxm = dxm; o.dym = dym; o.dzm = dzm;
         o.exm = exm; o.eym = eym; o.ezm = ezm;
-        if (load_id) o.id = __builtin_nontemporal_load(ids + ob / ES);
+        if (load_id) o.id = ids[ob / ES];
         o.zs = ZS && zs;
         // halo H^{n-1/2}: row (i, j0-1): Hx, Hz; column (i0-1 | i0+TXW, row cr): Hy, Hz
         o.hh0 = 0.f; o.hh1 = 0.f;

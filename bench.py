@@ -622,6 +622,13 @@ def bench_batched(a):
     from fdtd_amd import hip
     from fdtd_amd import workloads as W
 
+    fixed = None
+    if a.shape:
+        # a fixed shape (counter passes need the same kernel in every pass): 64 x tile rows x planes
+        tx, fty, fkc = (int(v) for v in a.shape.lower().split("x"))
+        if tx != 64:
+            raise SystemExit("batched: --shape takes 64xTYxKC (64-wide tiles)")
+        fixed = (fty, fkc)
     ws = W.batched_set()
     insts = [_instance(F, w, a.storage, int(a.load_zero_terms)) for w in ws]
     work = sum(w.cells * w.steps for w in ws)
@@ -634,11 +641,8 @@ def bench_batched(a):
     t_upload = time.perf_counter() - t0
     stream = hip.Stream()
     t0 = time.perf_counter()
-    if a.shape:
-        # a fixed shape (counter passes need the same kernel in every pass): 64 x tile rows x planes
-        tx, ty, kc = (int(v) for v in a.shape.lower().split("x"))
-        if tx != 64:
-            raise SystemExit("batched: --shape takes 64xTYxKC (64-wide tiles)")
+    if fixed:
+        ty, kc = fixed
         b.set_shape(tile_y=ty, kchunk=kc)
     else:
         ty, kc = b.autotune(trial_steps=4, stream=stream.handle)

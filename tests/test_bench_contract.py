@@ -20,6 +20,14 @@ def test_more_than_one_gpu_is_refused():
     assert out["n_gpus"] == 2 and "one GPU" in out["error"]
 
 
+def test_batched_fixed_shape_takes_64_wide_tiles():
+    """--shape for the batch fixes tile rows and planes of its 64-wide tiles;
+    any other width is refused before the device is touched."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "batched", "--shape", "128x2x6"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "64xTYxKC" in r.stderr
+
+
 def test_profiled_traffic_matches_the_launch_shape():
     """Each figure comes from the section of its own workload, launch shape,
     block order and kernel form (zero-term rows, or every row loading),
